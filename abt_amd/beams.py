@@ -1,0 +1,103 @@
+"""Beam-set container: the flattened form of PBT's outgoing-beam trees.
+
+A beam set holds what `diffraction` (diff_mod.f90:446-672) reads from each
+`outbeamtype` record (types_mod.f90:8-17) plus the facet vertices it looks up
+through `fi` (diff_mod.f90:529-531):
+
+    ampl      (nb, 8)  float64  re/im of ampl(1,1), ampl(2,1), ampl(1,2), ampl(2,2)
+                                (Fortran column-major order of complex(8) ampl(2,2))
+    vk7       (nb, 3)  float64  e-perp vector
+    prop      (nb, 3)  float64  outgoing propagation vector (prop_out)
+    fov       (nb,)    float64  field of view (only read when fast_diff is on)
+    vert_off  (nb+1,)  int64    offsets into verts, in vertices
+    verts     (nv, 3)  float64  facet vertices, in the facet's own order
+
+The same layout crosses the C-ABI (include/abt_diff.h, struct abt_beams).
+A file form (`save`/`load`, numpy .npz, no pickle) keeps beam sets
+reproducible across hosts.
+"""
+from __future__ import annotations
+
+import dataclasses
+import numpy as np
+
+
+@dataclasses.dataclass
+class BeamSet:
+    ampl: np.ndarray
+    vk7: np.ndarray
+    prop: np.ndarray
+    fov: np.ndarray
+    vert_off: np.ndarray
+    verts: np.ndarray
+
+    def __post_init__(self):
+        self.ampl = np.ascontiguousarray(self.ampl, dtype=np.float64).reshape(-1, 8)
+        nb = self.ampl.shape[0]
+        self.vk7 = np.ascontiguousarray(self.vk7, dtype=np.float64).reshape(nb, 3)
+        self.prop = np.ascontiguousarray(self.prop, dtype=np.float64).reshape(nb, 3)
+        self.fov = np.ascontiguousarray(self.fov, dtype=np.float64).reshape(nb)
+        self.vert_off = np.ascontiguousarray(self.vert_off, dtype=np.int64).reshape(nb + 1)
+        self.verts = np.ascontiguousarray(self.verts, dtype=np.float64).reshape(-1, 3)
+        if self.vert_off[0] != 0 or self.vert_off[-1] != self.verts.shape[0]:
+            raise ValueError("vert_off must start at 0 and end at len(verts)")
+        if nb and np.any(np.diff(self.vert_off) < 3):
+            raise ValueError("every aperture needs at least 3 vertices")
+
+    @property
+    def n(self) -> int:
+        return self.ampl.shape[0]
+
+    @property
+    def nv(self) -> np.ndarray:
+        return np.diff(self.vert_off)
+
+    @staticmethod
+    def empty() -> "BeamSet":
+        return BeamSet(np.zeros((0, 8)), np.zeros((0, 3)), np.zeros((0, 3)), np.zeros(0),
+                       np.zeros(1, np.int64), np.zeros((0, 3)))
+
+    @staticmethod
+    def from_triangles(ampl, vk7, prop, fov, tri_verts) -> "BeamSet":
+        """tri_verts: (nb, 3, 3) vertex coordinates of triangular apertures."""
+        tri_verts = np.asarray(tri_verts, dtype=np.float64)
+        nb = tri_verts.shape[0]
+        return BeamSet(ampl, vk7, prop, fov, np.arange(nb + 1, dtype=np.int64) * 3,
+                       tri_verts.reshape(-1, 3))
+
+    def ampl_complex(self) -> np.ndarray:
+        """(nb, 2, 2) complex view in row/column order ampl[b, r, c]."""
+        a = self.ampl.reshape(-1, 2, 2, 2)  # [b, c, r, re/im]
+        return (a[..., 0] + 1j * a[..., 1]).transpose(0, 2, 1)
+
+    def subset(self, idx) -> "BeamSet":
+        idx = np.asarray(idx, dtype=np.int64)
+        nv = self.nv[idx]
+        off = np.zeros(len(idx) + 1, np.int64)
+        np.cumsum(nv, out=off[1:])
+        verts = np.concatenate([self.verts[self.vert_off[i]:self.vert_off[i + 1]] for i in idx]) \
+            if len(idx) else np.zeros((0, 3))
+        return BeamSet(self.ampl[idx], self.vk7[idx], self.prop[idx], self.fov[idx], off, verts)
+
+    def save(self, path):
+        np.savez_compressed(path, ampl=self.ampl, vk7=self.vk7, prop=self.prop, fov=self.fov,
+                            vert_off=self.vert_off, verts=self.verts)
+
+    @staticmethod
+    def load(path) -> "BeamSet":
+        with np.load(path, allow_pickle=False) as z:
+            return BeamSet(z["ampl"], z["vk7"], z["prop"], z["fov"], z["vert_off"], z["verts"])
+
+
+def concat(sets) -> BeamSet:
+    sets = list(sets)
+    if not sets:
+        return BeamSet.empty()
+    off = [np.zeros(1, np.int64)]
+    base = 0
+    for s in sets:
+        off.append(s.vert_off[1:] + base)
+        base += s.verts.shape[0]
+    return BeamSet(np.concatenate([s.ampl for s in sets]), np.concatenate([s.vk7 for s in sets]),
+                   np.concatenate([s.prop for s in sets]), np.concatenate([s.fov for s in sets]),
+                   np.concatenate(off), np.concatenate([s.verts for s in sets]))
