@@ -1,0 +1,146 @@
+// far_field.hip -- coherent far-field sum of prepared beams onto the (theta, phi) bins.
+//
+// Work decomposition (both kernels): one lane per far-field bin, bins contiguous
+// with phi fastest (the Fortran (nphi, ntheta) layout), 256 lanes per workgroup.
+// Beam records are staged in LDS one tile at a time (cooperative coalesced
+// loads, then broadcast reads) and every lane walks the tile in beam order, so
+// each bin's sum is formed in a fixed order and is run-to-run deterministic.
+// blockIdx.y splits the beam list into chunks when there are too few bin tiles
+// to fill the 256 CUs; chunk partials are then summed in chunk order by
+// k_reduce_chunks.
+//
+// k_ff_fp64  -- the reference's formulas in FP64, operation for operation
+//               (diffraction, diff_mod.f90:576-670; karczewski :124-198;
+//               contour_integral :357-421), FMA contraction off.
+// k_ff_mixed -- the same quantity restructured for the vector ALUs:
+//               * geometry, phases and their reduction to octants in FP64
+//                 (the bin direction, q = k (prop - khat), vertex phases,
+//                 the k|r - com| ~ 1.2e7 rad far-field phase);
+//               * polarisation (Karczewski-Wolf, scattering-plane rotation)
+//                 and the aperture integral body in FP32: the edge sum is the
+//                 rotation-invariant polygon form factor
+//                   sum_ref = -(1/2pi) sum_j (qx dy_j - qy dx_j)/|q|^2 sinc(K_j/2) e^{i mu_j}
+//                 (q in revolutions/um), which equals the reference's split
+//                 kxx/kyy sum but is singular only at q -> 0;
+//               * near q -> 0 (|q| a < SERIES_QA) an FP64 power series of the
+//                 form factor (complete homogeneous symmetric polynomials of
+//                 the vertex phases);
+//               * the scattering-plane rotation temp_rot1 depends on phi only,
+//                 so it is applied once per bin after the beam sum; the
+//                 polarisation matrix rot4 . diff_ampl has the form g I + h J
+//                 and is folded into two complex weights per beam;
+//               * accumulation in FP64.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#include "ffbody.h"
+
+namespace abt {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+template <int PREC, int TB>
+__global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __restrict__ recs,
+                                                     const double* __restrict__ vrecs,
+                                                     int64_t nb, int64_t per_chunk, int fast_diff,
+                                                     double* __restrict__ out,
+                                                     double* __restrict__ work)
+{
+    __shared__ double tile[TB * REC];
+    const int tid = threadIdx.x;
+    const int bin = blockIdx.x * BLOCK + tid;
+    const bool active = bin < g.nbins;
+    const int bc = active ? bin : g.nbins - 1;
+    const int iphi = bc % g.nphi;
+    BinIn bi;
+    bi.x = g.x[bc]; bi.y = g.y[bc]; bi.z = g.z[bc];
+    const int bt = iphi + g.nphi * g.theta_i;
+    bi.xt = g.x[bt]; bi.yt = g.y[bt]; bi.zt = g.z[bt];
+    const double cp = g.cphi[iphi], sp = g.sphi[iphi];
+    double acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) acc[q] = 0.0;
+    const int64_t b0 = (int64_t)blockIdx.y * per_chunk;
+    const int64_t b1 = min(nb, b0 + per_chunk);
+    for (int64_t t0 = b0; t0 < b1; t0 += TB) {
+        const int nt = (int)min((int64_t)TB, b1 - t0);
+        __syncthreads();
+        const double* src = recs + t0 * REC;
+        for (int q = tid; q < nt * REC; q += BLOCK) tile[q] = src[q];
+        __syncthreads();
+        if (active) {
+            for (int bb = 0; bb < nt; bb++) {
+                if (PREC == 0)
+                    beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc);
+                else
+                    beam_bin_mixed(tile + bb * REC, vrecs, g, bi, fast_diff, acc);
+            }
+        }
+    }
+    if (!active) return;
+    if (PREC != 0) apply_temp_rot1(acc, cp, sp);
+    const int nbins = g.nbins;
+    if (work == nullptr) {
+        double2* o2 = reinterpret_cast<double2*>(out);
+#pragma unroll
+        for (int q = 0; q < 4; q++) o2[bin + (int64_t)nbins * q] = make_double2(acc[2 * q], acc[2 * q + 1]);
+    } else {
+        double* wc = work + (int64_t)blockIdx.y * 8 * nbins;
+#pragma unroll
+        for (int q = 0; q < 8; q++) wc[(int64_t)q * nbins + bin] = acc[q];
+    }
+}
+
+// Sum chunk partials in chunk order: work[(c*8 + comp)*nbins + bin] -> out.
+__global__ __launch_bounds__(BLOCK) void k_reduce_chunks(const double* __restrict__ work,
+                                                         int nchunks, int nbins,
+                                                         double* __restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * BLOCK + threadIdx.x;   // over 8*nbins
+    if (idx >= (int64_t)8 * nbins) return;
+    const int comp = (int)(idx / nbins);
+    const int bin = (int)(idx - (int64_t)comp * nbins);
+    double s = 0.0;
+    for (int c = 0; c < nchunks; c++) s = s + work[((int64_t)c * 8 + comp) * nbins + bin];
+    const int q = comp >> 1, ri = comp & 1;
+    out[2 * (bin + (int64_t)nbins * q) + ri] = s;
+}
+
+}  // namespace abt
+
+// ---------------------------------------------------------------------------
+// launch helpers (called by the runtime)
+// Zero-fill of an output (no hipMemsetAsync, so callers may capture it in a graph).
+extern "C" int abt_launch_zero(double* out, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(abt::k_reduce_chunks, dim3((unsigned)((n + abt::BLOCK - 1) / abt::BLOCK)),
+                       dim3(abt::BLOCK), 0, st, (const double*)nullptr, 0, (int)(n / 8), out);
+    return (int)hipGetLastError();
+}
+
+extern "C" int abt_launch_far_field(const abt::DevGrid* g, const double* recs, const double* vrecs,
+                                    int64_t nb, int prec, int tb, int fast_diff, int nchunks,
+                                    double* out, double* work, hipStream_t st)
+{
+    using namespace abt;
+    if (g->nbins <= 0) return 0;
+    const int64_t per_chunk = (nb + nchunks - 1) / nchunks;
+    dim3 grid((g->nbins + BLOCK - 1) / BLOCK, nchunks);
+    double* w = nchunks > 1 ? work : nullptr;
+#define L(P, T) hipLaunchKernelGGL((k_far_field<P, T>), grid, dim3(BLOCK), 0, st, *g, recs, vrecs, nb, per_chunk, fast_diff, out, w)
+    if (prec == 0) {
+        if (tb == 8) L(0, 8); else if (tb == 32) L(0, 32); else L(0, 16);
+    } else {
+        if (tb == 8) L(1, 8); else if (tb == 32) L(1, 32); else L(1, 16);
+    }
+#undef L
+    if (nchunks > 1) {
+        const int64_t n = (int64_t)8 * g->nbins;
+        hipLaunchKernelGGL(k_reduce_chunks, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                           st, work, nchunks, g->nbins, out);
+    }
+    return (int)hipGetLastError();
+}

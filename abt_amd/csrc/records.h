@@ -1,0 +1,70 @@
+// records.h -- device layout of prepared beam records and the far-field grid.
+//
+// One beam record per outgoing beam: the per-beam frame quantities of
+// `diffraction` (diff_mod.f90:515-574) computed once in FP64, followed by the
+// aperture edges of triangular apertures inline.  Apertures with more than
+// three vertices keep their edges in a separate vertex-record array.
+#pragma once
+#include <stdint.h>
+
+namespace abt {
+
+// ---- beam record: REC doubles per beam --------------------------------------
+enum : int {
+    R_COM = 0,      // [3] aperture centre (vertex mean), global frame
+    R_RR = 3,       // [9] matmul(rot2, rot), row-major
+    R_PROP2 = 12,   // [3] propagation vector in the aperture frame
+    R_INC2 = 15,    // [3] incidence (0,0,-1) in the aperture frame
+    R_AMPL = 18,    // [8] amplitude matrix, sign-flipped when anti-parallel:
+                    //     re/im of (1,1) (2,1) (1,2) (2,2)
+    R_PROP0 = 26,   // [3] propagation vector, global frame
+    R_COSFOV = 29,  // cos(fov)
+    R_NV = 30,      // number of vertices
+    R_VOFF = 31,    // first vertex record of this beam (apertures with nv > 3)
+    R_KPROP = 32,   // [3] prop0 * k / (2 pi)   (revolutions per um)
+    R_CC = 35,      // |com|^2
+    R_AMAX = 36,    // max_j |v20_j|, aperture radius about com
+    R_AREA = 37,    // signed aperture area in the aperture frame (negative: clockwise)
+    R_FRAC = 38,    // 1 / sqrt(1 - prop2_y^2)
+    R_KY1 = 39,     // 1 - prop2_y^2
+    // inline edges for nv <= 3 (unused slots zero)
+    R_EX = 40,      // [3] aperture-frame x of vertex j
+    R_EY = 43,      // [3] aperture-frame y of vertex j
+    R_EM = 46,      // [3] slope m_j after the clamps of diff_mod.f90:379-382
+    R_EN = 49,      // [3] inverse slope n_j after the clamps
+    R_EDX = 52,     // [3] x_{j+1} - x_j
+    R_EDY = 55,     // [3] y_{j+1} - y_j
+    R_V20 = 58,     // [9] com-relative global vertices (x,y,z per vertex)
+    REC = 68        // padded to a multiple of 4 doubles (32 bytes)
+};
+
+// ---- vertex record (nv > 3): VREC doubles per vertex -------------------------
+enum : int {
+    V_X = 0, V_Y = 1, V_M = 2, V_N = 3, V_DX = 4, V_DY = 5, V_V20 = 6, VREC = 12
+};
+
+// ---- device grid ------------------------------------------------------------
+struct DevGrid {
+    int32_t nphi, ntheta, nbins, theta_i;   // theta_i: 0-based forward-fallback column
+    double lambda, waveno;                  // waveno = 2*pi_r4/lambda
+    double krev;                            // waveno / (2 pi), revolutions per um
+    float inv_lambda_f;                     // 1/lambda (FP32 body of the mixed kernel)
+    const double* x;                        // [nbins] far-field bin positions
+    const double* y;
+    const double* z;
+    const double* cphi;                     // [nphi] cos(phi), glibc
+    const double* sphi;                     // [nphi] sin(phi)
+    const double* rinv;                     // [nbins] 1/|x|
+    const double* kr_rev;                   // [nbins] k|x|/(2 pi)
+    const double* theta;                    // [ntheta] theta_vals, rad
+    const double* phi;                      // [nphi] phi_vals, rad
+};
+
+// REAL*4 literals of the reference, widened to double
+constexpr double PI_R4 = 3.1415927410125732;
+constexpr double K2CLAMP = 0.9999989867210388;
+constexpr double KMIN = 9.999999974752427e-07;
+constexpr double FWD_DOT = 0.9990000128746033;
+constexpr double ANTI_THR = 9.999999747378752e-05;
+
+}  // namespace abt

@@ -1,0 +1,265 @@
+"""ctypes binding of libabt_diff.so (include/abt_diff.h).
+
+The library is the product path: every call here goes to the HIP library, and
+loading fails loudly when it has not been built (there is no CPU fallback).
+Device entry points take raw device pointers; `torch` tensors are accepted and
+passed by `data_ptr()` on the current stream.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .beams import BeamSet
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libabt_diff.so")
+
+ABT_FP64 = 0
+ABT_MIXED = 1
+PRECISIONS = {"fp64": ABT_FP64, "mixed": ABT_MIXED, ABT_FP64: ABT_FP64, ABT_MIXED: ABT_MIXED}
+PI_R4 = float(np.float32(3.14159265358979))
+
+# every entry point declared in include/abt_diff.h
+EXPORTS = [
+    "abt_version", "abt_last_error", "abt_init", "abt_finalize", "abt_make_far_field_bins",
+    "abt_diff_main", "abt_diffraction", "abt_finalise", "abt_record_doubles",
+    "abt_vertex_record_doubles", "abt_prepare_beams_host", "abt_prepare_beams_dev",
+    "abt_grid_create", "abt_grid_destroy", "abt_grid_bins", "abt_far_field_workspace",
+    "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
+]
+
+
+class AbtError(RuntimeError):
+    pass
+
+
+class _Beams(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("ampl", ctypes.c_void_p), ("vk7", ctypes.c_void_p),
+                ("prop", ctypes.c_void_p), ("fov", ctypes.c_void_p), ("vert_off", ctypes.c_void_p),
+                ("verts", ctypes.c_void_p)]
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int64), ("n_phi", ctypes.c_int64),
+                ("theta", ctypes.c_void_p), ("phi", ctypes.c_void_p)]
+
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AbtError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       "(make -C abt_amd/csrc); there is no CPU fallback")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i64, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+    L.abt_version.restype = i
+    L.abt_last_error.restype = ctypes.c_char_p
+    L.abt_init.argtypes = [i]
+    L.abt_make_far_field_bins.argtypes = [i64, vp, i64, vp, vp, vp, vp]
+    L.abt_diff_main.argtypes = [ctypes.POINTER(_Grid), d, i, ctypes.POINTER(_Beams),
+                                ctypes.POINTER(_Beams), i, vp, vp]
+    L.abt_diffraction.argtypes = [i64, i64, vp, vp, vp, d, vp, vp, i, vp, vp, vp, d, i64, vp, i, vp]
+    L.abt_record_doubles.restype = i64
+    L.abt_vertex_record_doubles.restype = i64
+    L.abt_prepare_beams_host.argtypes = [ctypes.POINTER(_Beams), d, d, vp, vp]
+    L.abt_prepare_beams_dev.argtypes = [ctypes.POINTER(_Beams), d, d, vp, vp, vp]
+    L.abt_grid_create.argtypes = [ctypes.POINTER(_Grid), d, ctypes.POINTER(vp)]
+    L.abt_grid_destroy.argtypes = [vp]
+    L.abt_grid_bins.argtypes = [vp]
+    L.abt_grid_bins.restype = i64
+    L.abt_far_field_workspace.argtypes = [vp, i64, i]
+    L.abt_far_field_workspace.restype = ctypes.c_size_t
+    L.abt_far_field_dev.argtypes = [vp, vp, vp, i64, i, i, vp, vp, ctypes.c_size_t, vp]
+    L.abt_finalise.argtypes = [ctypes.POINTER(_Grid), d, vp, vp, i, d, d, vp, vp, vp, vp, vp,
+                               vp, vp]
+    L.abt_finalise_workspace.argtypes = [vp]
+    L.abt_finalise_workspace.restype = ctypes.c_size_t
+    L.abt_finalise_dev.argtypes = [vp, vp, vp, i, d, d, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().abt_last_error().decode(errors="replace")
+        raise AbtError(f"abt error {rc}: {msg}")
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def beams_struct(b: BeamSet, fov=True, keep=None):
+    s = _Beams(b.n, _ptr(b.ampl).value, _ptr(b.vk7).value, _ptr(b.prop).value,
+               _ptr(b.fov).value if fov else None, _ptr(b.vert_off).value, _ptr(b.verts).value)
+    return s
+
+
+def grid_struct(theta, phi):
+    return _Grid(len(theta), len(phi), _ptr(theta).value, _ptr(phi).value)
+
+
+def raw_to_complex(raw, nphi, nth):
+    """Fortran-layout (phi, theta, 2, 2) doubles -> complex [r, c, theta, phi]."""
+    a = np.asarray(raw).reshape(2, 2, nth, nphi, 2)
+    return (a[..., 0] + 1j * a[..., 1]).transpose(1, 0, 2, 3)
+
+
+# ---------------------------------------------------------------------------
+# reference interfaces (host arrays)
+
+def make_far_field_bins(theta, phi):
+    theta, phi = _f64(theta), _f64(phi)
+    n = len(theta) * len(phi)
+    x, y, z = np.empty(n), np.empty(n), np.empty(n)
+    _check(load().abt_make_far_field_bins(len(theta), _ptr(theta), len(phi), _ptr(phi), _ptr(x),
+                                          _ptr(y), _ptr(z)))
+    return x, y, z
+
+
+def diff_main(theta, phi, lam, beams: BeamSet, ext: BeamSet, precision="fp64", fast_diff=False,
+              raw=False):
+    """diff_main (diff_mod.f90:674-825): (ampl_far_beam, ampl_far_ext_diff)."""
+    theta, phi = _f64(theta), _f64(phi)
+    nb = len(theta) * len(phi)
+    ob, oe = np.empty(8 * nb), np.empty(8 * nb)
+    g = grid_struct(theta, phi)
+    _check(load().abt_diff_main(ctypes.byref(g), lam, int(fast_diff),
+                                ctypes.byref(beams_struct(beams)), ctypes.byref(beams_struct(ext)),
+                                PRECISIONS[precision], _ptr(ob), _ptr(oe)))
+    if raw:
+        return ob, oe
+    return raw_to_complex(ob, len(phi), len(theta)), raw_to_complex(oe, len(phi), len(theta))
+
+
+def diffraction(theta, phi, x, y, z, lam, beams: BeamSet, b, precision="fp64", fast_diff=False):
+    theta, phi = _f64(theta), _f64(phi)
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    out = np.empty(8 * len(theta) * len(phi))
+    o0, o1 = beams.vert_off[b], beams.vert_off[b + 1]
+    verts = _f64(beams.verts[o0:o1])
+    _check(load().abt_diffraction(len(phi), len(theta), _ptr(x), _ptr(y), _ptr(z), lam, _ptr(phi),
+                                  _ptr(theta), int(fast_diff), _ptr(_f64(beams.ampl[b])),
+                                  _ptr(_f64(beams.vk7[b])), _ptr(_f64(beams.prop[b])),
+                                  float(beams.fov[b]), int(o1 - o0), _ptr(verts),
+                                  PRECISIONS[precision], _ptr(out)))
+    return raw_to_complex(out, len(phi), len(theta))
+
+
+def prepare_beams_host(beams: BeamSet, lam, fov_value=None):
+    """Host set-up of the per-beam records: (recs (n, REC), vrecs (nvert, VREC))."""
+    L = load()
+    rec, vrec = L.abt_record_doubles(), L.abt_vertex_record_doubles()
+    recs = np.zeros((beams.n, rec))
+    vrecs = np.zeros((max(beams.verts.shape[0], 1), vrec))
+    s = beams_struct(beams, fov=fov_value is None)
+    _check(L.abt_prepare_beams_host(ctypes.byref(s), 0.0 if fov_value is None else fov_value, lam,
+                                    _ptr(recs), _ptr(vrecs)))
+    return recs, vrecs
+
+
+# ---------------------------------------------------------------------------
+# device-resident interface
+
+class DeviceGrid:
+    """Far-field bins resident on the device (abt_grid_create)."""
+
+    def __init__(self, theta, phi, lam):
+        self.theta, self.phi = _f64(theta), _f64(phi)
+        self.lam = float(lam)
+        h = ctypes.c_void_p()
+        g = grid_struct(self.theta, self.phi)
+        _check(load().abt_grid_create(ctypes.byref(g), self.lam, ctypes.byref(h)))
+        self.handle = h
+        self.nbins = int(load().abt_grid_bins(h))
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            load().abt_grid_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        try:
+            import torch
+            return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except Exception:
+            return None
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(int(stream))
+
+
+def prepare_beams_dev(dev_beams: dict, n, lam, recs, vrecs=None, fov_value=None, stream=None):
+    """dev_beams: dict of device tensors ampl, vk7, prop, (fov), vert_off, verts."""
+    s = _Beams(n, dev_beams["ampl"].data_ptr(), dev_beams["vk7"].data_ptr(),
+               dev_beams["prop"].data_ptr(),
+               dev_beams["fov"].data_ptr() if (fov_value is None and "fov" in dev_beams) else None,
+               dev_beams["vert_off"].data_ptr(), dev_beams["verts"].data_ptr())
+    _check(load().abt_prepare_beams_dev(ctypes.byref(s), 0.0 if fov_value is None else fov_value,
+                                        lam, _ptr(recs), _ptr(vrecs), _stream_ptr(stream)))
+
+
+def far_field_workspace(grid: DeviceGrid, nb, precision="fp64"):
+    return int(load().abt_far_field_workspace(grid.handle, nb, PRECISIONS[precision]))
+
+
+def far_field_dev(grid: DeviceGrid, recs, nb, out, vrecs=None, precision="fp64", fast_diff=False,
+                  work=None, stream=None):
+    wb = 0 if work is None else work.numel() * work.element_size()
+    _check(load().abt_far_field_dev(grid.handle, _ptr(recs), _ptr(vrecs), nb, PRECISIONS[precision],
+                                    int(fast_diff), _ptr(out), _ptr(work), wb, _stream_ptr(stream)))
+
+
+def finalise(theta, phi, lam, raw_beam, raw_ext, scaling=False, beam_energy_out=1.0,
+             ext_energy_out=1.0):
+    """finalise (outputs_mod.f90:63-278) on the device; raw_* are Fortran-layout
+    amplitude buffers (copied, then scaled by k like the reference)."""
+    theta, phi = _f64(theta), _f64(phi)
+    nth, nph = len(theta), len(phi)
+    nb = nth * nph
+    rb = np.array(raw_beam, dtype=np.float64, copy=True)
+    re = np.array(raw_ext, dtype=np.float64, copy=True)
+    m = [np.empty(16 * nb) for _ in range(3)]
+    m1 = [np.empty(16 * nth) for _ in range(3)]
+    params = np.zeros(9)
+    g = grid_struct(theta, phi)
+    _check(load().abt_finalise(ctypes.byref(g), lam, _ptr(rb), _ptr(re), int(scaling),
+                               beam_energy_out, ext_energy_out, _ptr(m[0]), _ptr(m[1]), _ptr(m[2]),
+                               _ptr(m1[0]), _ptr(m1[1]), _ptr(m1[2]), _ptr(params)))
+    keys = ["scatt", "scatt_beam", "scatt_ext_diff", "asymmetry", "asymmetry_beam",
+            "asymmetry_ext_diff", "back_scatt", "scale_beam", "scale_ext"]
+    return dict(mueller=m[0].reshape(16, nb), mueller_beam=m[1].reshape(16, nb),
+                mueller_ext=m[2].reshape(16, nb), mueller_1d=m1[0].reshape(16, nth),
+                mueller_1d_beam=m1[1].reshape(16, nth), mueller_1d_ext=m1[2].reshape(16, nth),
+                ampl_beam=rb, ampl_ext=re, **dict(zip(keys, params)))
+
+
+def finalise_dev(grid: DeviceGrid, ab, ae, mueller, m1d, params, scaling=False,
+                 beam_energy_out=1.0, ext_energy_out=1.0, work=None, stream=None):
+    wb = 0 if work is None else work.numel() * work.element_size()
+    _check(load().abt_finalise_dev(grid.handle, _ptr(ab), _ptr(ae), int(scaling), beam_energy_out,
+                                   ext_energy_out, _ptr(mueller), _ptr(m1d), _ptr(params),
+                                   _ptr(work), wb, _stream_ptr(stream)))

@@ -89,14 +89,16 @@ def random_triangles(n, seed=0, box=10.0, size=(0.5, 4.0), amp_decades=0.0) -> B
 def stress_beams(n=200_000, seed=20240601) -> BeamSet:
     """Diffraction stress test: n planar triangles, centroids uniform in a 100 um cube,
     two edge vectors of length U(0.3, 5) um in the plane normal to the propagation
-    vector (uniform on the sphere), amplitude entries (N(0,1) + i N(0,1)) * 10^U(-6,0),
-    e-perp a seeded unit vector normal to prop.  Generator: numpy PCG64 with `seed`."""
+    vector (uniform on the sphere) at an angle in [0.3, pi - 0.3] rad to each other,
+    amplitude entries (N(0,1) + i N(0,1)) * 10^U(-6,0), e-perp a seeded unit vector
+    normal to prop.  Generator: numpy PCG64 with `seed`."""
     rng = np.random.default_rng(seed)
     c = rng.uniform(0.0, 100.0, (n, 3))
     prop = _unit(rng.normal(size=(n, 3)))
     a, b = _perp_basis(prop, rng)
     ang1 = rng.uniform(0, 2 * np.pi, (n, 1))
-    ang2 = ang1 + rng.uniform(0.3, 2 * np.pi - 0.3, (n, 1))  # keep the two edges non-parallel
+    # angle between the two edges in [0.3, pi - 0.3] rad either way round: no sliver triangles
+    ang2 = ang1 + rng.uniform(0.3, np.pi - 0.3, (n, 1)) * rng.choice([-1.0, 1.0], (n, 1))
     l1 = rng.uniform(0.3, 5.0, (n, 1))
     l2 = rng.uniform(0.3, 5.0, (n, 1))
     e1 = l1 * (np.cos(ang1) * a + np.sin(ang1) * b)

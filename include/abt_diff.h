@@ -1,0 +1,163 @@
+/*
+ * abt_diff.h -- C-ABI of the MI355X far-field diffraction library (libabt_diff.so).
+ *
+ * Drop-in boundary for PBT's far-field diffraction module `diff_mod`
+ * (src/diff_mod.f90 of hballington12/abt).  The Fortran module
+ * abt_amd/fortran/diff_mod.f90 keeps the reference's procedure interfaces and
+ * reaches this library through iso_c_binding; the same entry points serve the
+ * Python/PyTorch harness.
+ *
+ * Conventions
+ *  - Return value: 0 on success, nonzero on error; abt_last_error() then holds a
+ *    thread-local message.
+ *  - Host entry points take host pointers and are synchronous.  "_dev" entry
+ *    points take device pointers and a HIP stream (hipStream_t passed as void*;
+ *    NULL = the default stream) and are asynchronous on that stream.
+ *  - Amplitude arrays use the Fortran layout complex(8) a(nphi, ntheta, 2, 2):
+ *    interleaved (re, im) doubles, bin b = i_phi + nphi * j_theta, element (r, c)
+ *    (0-based) at complex index b + nbins * (r + 2 c).
+ *  - Mueller arrays: real(8) m(nphi, ntheta, 16) and m1d(ntheta, 16).
+ *  - Angles in radians.  Lengths in micrometres (the caller's unit).
+ */
+#ifndef ABT_DIFF_H
+#define ABT_DIFF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ABT_DIFF_ABI_VERSION 1
+
+enum abt_status {
+    ABT_OK = 0,
+    ABT_ERR_ARG = 1,      /* invalid argument or shape */
+    ABT_ERR_HIP = 2,      /* HIP runtime error */
+    ABT_ERR_NODEV = 3,    /* no usable gfx950 device */
+    ABT_ERR_NOMEM = 4     /* device allocation failed */
+};
+
+enum abt_precision {
+    ABT_FP64 = 0,         /* plain FP64 kernel, the reference's formulas */
+    ABT_MIXED = 1         /* FP32 body, FP64 phases, FP64 series near the forward limit */
+};
+
+/* One tree of outgoing beams (the flattened outbeamtype records of
+ * types_mod.f90:8-17 plus the facet vertices `diffraction` looks up through
+ * `fi`, diff_mod.f90:529-531). */
+typedef struct abt_beams {
+    int64_t n;                /* number of beams */
+    const double* ampl;       /* [n*8]  re/im of ampl(1,1), ampl(2,1), ampl(1,2), ampl(2,2) */
+    const double* vk7;        /* [n*3]  e-perp vector */
+    const double* prop;       /* [n*3]  outgoing propagation vector (prop_out) */
+    const double* fov;        /* [n]    field of view, rad (NULL: every beam uses fov_value) */
+    const int64_t* vert_off;  /* [n+1]  vertex offsets; vertices of beam b: vert_off[b]..vert_off[b+1]-1 */
+    const double* verts;      /* [3*vert_off[n]]  x,y,z per vertex */
+} abt_beams;
+
+/* Angular grid: theta_vals / phi_vals of job_parameters_type (types_mod.f90:73-74). */
+typedef struct abt_grid {
+    int64_t n_theta;
+    int64_t n_phi;
+    const double* theta;      /* [n_theta] rad */
+    const double* phi;        /* [n_phi]   rad */
+} abt_grid;
+
+typedef struct abt_dev_grid abt_dev_grid;   /* opaque: far-field bins resident on the device */
+
+/* ---- library ------------------------------------------------------------ */
+int abt_version(void);
+const char* abt_last_error(void);
+/* Select the device for the calling process (lazy: host entry points call it
+ * with device 0 when it was never called). */
+int abt_init(int device);
+/* Free every device buffer of the library (grids created by the caller stay
+ * valid until abt_grid_destroy). */
+int abt_finalize(void);
+
+/* ---- reference interfaces ------------------------------------------------ */
+
+/* make_far_field_bins (diff_mod.f90:827-855): x,y,z of shape (n_phi, n_theta). */
+int abt_make_far_field_bins(int64_t n_theta, const double* theta, int64_t n_phi,
+                            const double* phi, double* x, double* y, double* z);
+
+/* diff_main (diff_mod.f90:674-825).  Sums every beam of `beams` into
+ * ampl_far_beam and every beam of `ext` (fov forced to pi/2, diff_mod.f90:797)
+ * into ampl_far_ext_diff, both overwritten.  Beams are summed in tree order
+ * for every bin (deterministic).  is_fast_diff: FOV culling (diff_mod.f90:593-603). */
+int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff,
+                  const abt_beams* beams, const abt_beams* ext, int precision,
+                  double* ampl_far_beam, double* ampl_far_ext_diff);
+
+/* diffraction (diff_mod.f90:446-672) for one beam on caller-given bins
+ * xfar/yfar/zfar (n_phi, n_theta); amplC overwritten.  Re-entrant: callable
+ * concurrently from several host threads (the reference's -memory mode calls it
+ * inside OpenMP regions, beam_loop_mod.f90:37-51, :1935-1967). */
+int abt_diffraction(int64_t n_phi, int64_t n_theta, const double* xfar, const double* yfar,
+                    const double* zfar, double lambda, const double* phi_vals,
+                    const double* theta_vals, int is_fast_diff, const double* ampl,
+                    const double* vk7, const double* prop, double fov, int64_t nv,
+                    const double* verts, int precision, double* amplC);
+
+/* finalise (outputs_mod.f90:63-278), numerical part, on the device.  Inputs are
+ * diff_main's outputs without the k factor; they are scaled by k in place like
+ * the reference.  Any output pointer may be NULL.  params[9] = scatt, scatt_beam,
+ * scatt_ext_diff, asymmetry, asymmetry_beam, asymmetry_ext_diff, back_scatt,
+ * beam scale, ext scale (the last two 0 unless scaling). */
+int abt_finalise(const abt_grid* grid, double lambda, double* ampl_far_beam,
+                 double* ampl_far_ext_diff, int scaling, double beam_energy_out,
+                 double ext_energy_out, double* mueller, double* mueller_beam,
+                 double* mueller_ext, double* mueller_1d, double* mueller_1d_beam,
+                 double* mueller_1d_ext, double* params);
+
+/* ---- device-resident interface (no reference counterpart: the resident form of
+ * diff_main for callers whose beams already live in HBM) ------------------------ */
+
+/* Doubles per prepared beam record / per vertex record (see abt_amd/csrc/records.h). */
+int64_t abt_record_doubles(void);
+int64_t abt_vertex_record_doubles(void);
+
+/* Per-beam frame set-up of `diffraction` (diff_mod.f90:515-574) on the host, in
+ * the reference's operation order with glibc libm (bit-reproducible against a
+ * CPU FP64 run, including at normal incidence where the frame is set by
+ * rounding).  fov_value replaces beams->fov when that is NULL.  recs: n *
+ * abt_record_doubles(); vrecs: vert_off[n] * abt_vertex_record_doubles() (may be
+ * NULL when every aperture is a triangle). */
+int abt_prepare_beams_host(const abt_beams* beams, double fov_value, double lambda,
+                           double* recs, double* vrecs);
+
+/* The same set-up on the device from device-resident beams (FP64, device libm). */
+int abt_prepare_beams_dev(const abt_beams* dev_beams, double fov_value, double lambda,
+                          double* dev_recs, double* dev_vrecs, void* stream);
+
+/* Upload a grid: bins built on the host by make_far_field_bins. */
+int abt_grid_create(const abt_grid* grid, double lambda, abt_dev_grid** out);
+int abt_grid_destroy(abt_dev_grid* g);
+int64_t abt_grid_bins(const abt_dev_grid* g);
+
+/* Workspace bytes abt_far_field_dev needs for nb beams (0 when none). */
+size_t abt_far_field_workspace(const abt_dev_grid* g, int64_t nb, int precision);
+
+/* Coherent far-field sum of nb prepared beams: dev_out (complex, Fortran layout,
+ * 8*nbins doubles) overwritten.  dev_work: abt_far_field_workspace bytes (NULL:
+ * the library allocates). */
+int abt_far_field_dev(const abt_dev_grid* g, const double* dev_recs, const double* dev_vrecs,
+                      int64_t nb, int precision, int is_fast_diff, double* dev_out,
+                      void* dev_work, size_t work_bytes, void* stream);
+
+/* Device form of abt_finalise: amplitudes resident on the device (scaled by k in
+ * place).  dev_mueller: 3*16*nbins (total, beam, ext, each (nphi, ntheta, 16));
+ * dev_mueller_1d: 3*16*n_theta; dev_params: 9 doubles (as abt_finalise).
+ * dev_work: abt_finalise_workspace bytes (NULL: the library allocates). */
+size_t abt_finalise_workspace(const abt_dev_grid* g);
+int abt_finalise_dev(const abt_dev_grid* g, double* dev_ampl_beam, double* dev_ampl_ext,
+                     int scaling, double beam_energy_out, double ext_energy_out,
+                     double* dev_mueller, double* dev_mueller_1d, double* dev_params,
+                     void* dev_work, size_t work_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ABT_DIFF_H */

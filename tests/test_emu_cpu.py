@@ -1,0 +1,63 @@
+"""CPU checks of the kernel bodies (abt_amd/csrc/ffbody.h) through the host
+emulator: the FP64 body reproduces the oracle and the mixed-precision body
+meets the same criteria as on the GPU.  No GPU needed."""
+import numpy as np
+import pytest
+
+import oracle as O
+from abt_amd import lib
+from abt_amd.beams import BeamSet
+from abt_amd.workloads import grid, random_triangles, stress_beams
+from tests import emu
+from tests.gpu_util import ampl_rel_err, summary
+from tests.test_gpu_parity import (LAM, _check_fp64, _coarse, _column_case, _forward_mask,
+                                   check_mixed)
+
+
+def _run(prec, th, ph, bs, fov=None):
+    recs, vr = lib.prepare_beams_host(bs, LAM, fov_value=fov)
+    return emu.far_field(prec, th, ph, LAM, recs, bs.n, vr)
+
+
+def test_emu_fp64_body_matches_oracle(orc):
+    th, ph = _coarse()
+    b, e = _column_case()
+    rb, re = orc.diff_main(th, ph, LAM, b, e, nthreads=8)
+    _check_fp64(_run(0, th, ph, b), rb, _forward_mask(th, ph, [b]))
+    _check_fp64(_run(0, th, ph, e, O.PI_R4 / 2), re, _forward_mask(th, ph, [e]))
+
+
+def test_emu_mixed_body_meets_criteria(orc):
+    th, ph = _coarse()
+    b, e = _column_case()
+    rb, re = orc.diff_main(th, ph, LAM, b, e, nthreads=8)
+    check_mixed(_run(1, th, ph, b), rb, _forward_mask(th, ph, [b]))
+    check_mixed(_run(1, th, ph, e, O.PI_R4 / 2), re, _forward_mask(th, ph, [e]))
+    th, ph = grid((0, 2, 180), (0, 6, 354))
+    for bs in (stress_beams(150, seed=7), random_triangles(200, seed=4, box=40.0)):
+        ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=8)
+        check_mixed(_run(1, th, ph, bs), ref, _forward_mask(th, ph, [bs]))
+
+
+def test_emu_mixed_quadrilaterals(orc):
+    from tests.test_gpu_parity import test_fp64_quadrilateral_apertures  # noqa: F401
+    th, ph = _coarse()
+    rng = np.random.default_rng(8)
+    sets = []
+    for i in range(5):
+        c = rng.uniform(-5, 5, 3)
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        a = np.cross(n, rng.normal(size=3)); a /= np.linalg.norm(a)
+        bb = np.cross(n, a)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, 5))
+        rad = rng.uniform(1, 3, 5)
+        verts = c + (rad * np.cos(ang))[:, None] * a + (rad * np.sin(ang))[:, None] * bb
+        prop = n + 0.4 * a; prop /= np.linalg.norm(prop)
+        vk7 = np.cross(prop, a); vk7 /= np.linalg.norm(vk7)
+        sets.append(BeamSet(rng.normal(size=(1, 8)), vk7, prop, [1.0], [0, 5], verts))
+    from abt_amd.beams import concat
+    bs = concat(sets)
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=8)
+    got = _run(1, th, ph, bs)
+    err = ampl_rel_err(got, ref)
+    assert err.max() <= 2e-6, summary(err)
