@@ -57,6 +57,13 @@ def load():
     if not os.path.exists(LIB_PATH):
         raise AbtError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                        "(make -C abt_amd/csrc); there is no CPU fallback")
+    # One HIP runtime per process: when torch is present its libamdhip64 (same soname)
+    # must be loaded first so that the library binds to it instead of loading a second
+    # runtime that would then hide the GPU from torch.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     vp, i64, d, i = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
     L.abt_version.restype = i
