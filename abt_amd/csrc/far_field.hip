@@ -74,14 +74,20 @@ __global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __
             for (int bb = 0; bb < nt; bb++) {
                 if (PREC == 0)
                     beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc);
-                else
+                else if (PREC == 1)
                     beam_bin_mixed(tile + bb * REC, vrecs, g, bi, fast_diff, acc);
+                else
+                    beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, true);
             }
         }
     }
     if (!active) return;
-    if (PREC != 0) apply_temp_rot1(acc, cp, sp);
+    if (PREC == 1) mixed_finish(acc, cp, sp);
     const int nbins = g.nbins;
+    if (PREC == 2) {   // incoherent intensity: one double per bin, never chunked
+        out[bin] = acc[0];
+        return;
+    }
     if (work == nullptr) {
         double2* o2 = reinterpret_cast<double2*>(out);
 #pragma unroll
@@ -112,12 +118,20 @@ __global__ __launch_bounds__(BLOCK) void k_reduce_chunks(const double* __restric
 
 // ---------------------------------------------------------------------------
 // launch helpers (called by the runtime)
-// Zero-fill of an output (no hipMemsetAsync, so callers may capture it in a graph).
+namespace abt {
+__global__ __launch_bounds__(BLOCK) void k_fill_zero(double* __restrict__ out, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) out[i] = 0.0;
+}
+}  // namespace abt
+
+// Zero-fill of n doubles (a kernel, not hipMemsetAsync, so callers may capture it in a graph).
 extern "C" int abt_launch_zero(double* out, int64_t n, hipStream_t st)
 {
     if (n <= 0) return 0;
-    hipLaunchKernelGGL(abt::k_reduce_chunks, dim3((unsigned)((n + abt::BLOCK - 1) / abt::BLOCK)),
-                       dim3(abt::BLOCK), 0, st, (const double*)nullptr, 0, (int)(n / 8), out);
+    hipLaunchKernelGGL(abt::k_fill_zero, dim3((unsigned)((n + abt::BLOCK - 1) / abt::BLOCK)),
+                       dim3(abt::BLOCK), 0, st, out, n);
     return (int)hipGetLastError();
 }
 
@@ -133,8 +147,10 @@ extern "C" int abt_launch_far_field(const abt::DevGrid* g, const double* recs, c
 #define L(P, T) hipLaunchKernelGGL((k_far_field<P, T>), grid, dim3(BLOCK), 0, st, *g, recs, vrecs, nb, per_chunk, fast_diff, out, w)
     if (prec == 0) {
         if (tb == 8) L(0, 8); else if (tb == 32) L(0, 32); else L(0, 16);
-    } else {
+    } else if (prec == 1) {
         if (tb == 8) L(1, 8); else if (tb == 32) L(1, 32); else L(1, 16);
+    } else {
+        L(2, 16);
     }
 #undef L
     if (nchunks > 1) {

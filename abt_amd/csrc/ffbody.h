@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "records.h"
 #include "trig.h"
@@ -26,12 +27,22 @@ struct BinIn {
     double xt, yt, zt;   // bin (same phi, theta_i): forward fallback of the hc vector
 };
 
+__host__ __device__ __forceinline__ float abt_rcpf(float x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+
 // ---------------------------------------------------------------------------
 // FP64 plain kernel body: one beam, one bin.  acc: re/im of (1,1) (2,1) (1,2) (2,2)
 __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict__ r,
                                               const double* __restrict__ vrecs,
                                               const DevGrid& g, const BinIn& bi, double cp,
-                                              double sp, int fast_diff, double acc[8])
+                                              double sp, int fast_diff, double acc[8],
+                                              bool incoherent = false)
 {
 #pragma clang fp contract(off)
     const double x1 = bi.x - r[R_COM], y1 = bi.y - r[R_COM + 1], z1 = bi.z - r[R_COM + 2];
@@ -129,10 +140,20 @@ __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict_
         ar = ar + (cb * sumre - sb * sumim) / lam;
         ai = ai + (cb * sumim + sb * sumre) / lam;
     }
-    acc[0] = acc[0] + (c11r * ar - c11i * ai); acc[1] = acc[1] + (c11r * ai + c11i * ar);
-    acc[2] = acc[2] + (c21r * ar - c21i * ai); acc[3] = acc[3] + (c21r * ai + c21i * ar);
-    acc[4] = acc[4] + (c12r * ar - c12i * ai); acc[5] = acc[5] + (c12r * ai + c12i * ar);
-    acc[6] = acc[6] + (c22r * ar - c22i * ai); acc[7] = acc[7] + (c22r * ai + c22i * ar);
+    const double o0 = c11r * ar - c11i * ai, o1 = c11r * ai + c11i * ar;
+    const double o2 = c21r * ar - c21i * ai, o3 = c21r * ai + c21i * ar;
+    const double o4 = c12r * ar - c12i * ai, o5 = c12r * ai + c12i * ar;
+    const double o6 = c22r * ar - c22i * ai, o7 = c22r * ai + c22i * ar;
+    if (incoherent) {
+        // diagnostic: sum over beams of |amplC|_F^2 (conditioning of the coherent sum)
+        acc[0] = acc[0] + ((((o0 * o0 + o1 * o1) + (o2 * o2 + o3 * o3)) + (o4 * o4 + o5 * o5)) +
+                           (o6 * o6 + o7 * o7));
+        return;
+    }
+    acc[0] = acc[0] + o0; acc[1] = acc[1] + o1;
+    acc[2] = acc[2] + o2; acc[3] = acc[3] + o3;
+    acc[4] = acc[4] + o4; acc[5] = acc[5] + o5;
+    acc[6] = acc[6] + o6; acc[7] = acc[7] + o7;
 }
 
 // ---------------------------------------------------------------------------
@@ -157,26 +178,56 @@ __host__ __device__ __forceinline__ void sincos_rev64(double t, float* s, float*
     *c = co;
 }
 
-// sinc(pi*hh) = sin(pi*hh)/(pi*hh) ... expressed in revolutions: for an edge with
-// half-phase h (revolutions), returns sin(2 pi h) / (2 pi h), and e^{2 pi i mu}.
-__host__ __device__ __forceinline__ float sinc_rev64(double h)
+// Half-period reduction of a revolution count t (FP64): t = r + m/2 with
+// |r| <= 1/4, so that e^{2 pi i t} = (-1)^m e^{2 pi i r}.  The parity of m is
+// returned as an FP32 sign mask (bit 31), which callers fold into one product.
+__host__ __device__ __forceinline__ float half_reduce(double t, unsigned* sgn)
 {
-    const double q = rint(4.0 * h);
-    const float r = (float)fma(-0.25, q, h);
-    const float z = r * r;
-    const float ps = fmaf(z, fmaf(z, fmaf(z, fmaf(z, ABT_SR4, ABT_SR3), ABT_SR2), ABT_SR1), ABT_SR0);
-    const float cs = fmaf(z, fmaf(z, fmaf(z, fmaf(z, ABT_CR4, ABT_CR3), ABT_CR2), ABT_CR1), ABT_CR0);
-    const int qi = ((int)q) & 3;
-    float sn = (qi & 1) ? cs : r * ps;
-    sn = (qi >= 2) ? -sn : sn;
-    const float hf = (float)h;
-    // |h| <= 1/8: sin(2 pi r)/(2 pi r) = ps / (2 pi) exactly, no cancellation
+    const double m = rint(2.0 * t);
+    *sgn = ((unsigned)(int)m) << 31;
+    return (float)fma(-0.5, m, t);
+}
+
+__host__ __device__ __forceinline__ float flipsign(float x, unsigned sgn)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __uint_as_float(__float_as_uint(x) ^ sgn);
+#else
+    unsigned u;
+    memcpy(&u, &x, 4);
+    u ^= sgn;
+    memcpy(&x, &u, 4);
+    return x;
+#endif
+}
+
+// sin(2 pi r) / r and cos(2 pi r) for |r| <= 1/4
+__host__ __device__ __forceinline__ float sin_over_r(float z)
+{
+    return fmaf(z, fmaf(z, fmaf(z, fmaf(z, ABT_HR4, ABT_HR3), ABT_HR2), ABT_HR1), ABT_HR0);
+}
+__host__ __device__ __forceinline__ float cos_poly(float z)
+{
+    return fmaf(z, fmaf(z, fmaf(z, fmaf(z, fmaf(z, ABT_KR5, ABT_KR4), ABT_KR3), ABT_KR2), ABT_KR1),
+                ABT_KR0);
+}
+
+// sinc of an edge, sin(2 pi h) / (2 pi h) for the half phase difference h
+// (revolutions, FP64), up to the sign (-1)^m returned in *sgn.  For m = 0
+// (|h| <= 1/4) the quotient is P(r^2) / (2 pi) with no cancellation.
+__host__ __device__ __forceinline__ float sinc_half(double h, unsigned* sgn)
+{
+    const double m = rint(2.0 * h);
+    *sgn = ((unsigned)(int)m) << 31;
+    const float r = (float)fma(-0.5, m, h);
+    const float ps = sin_over_r(r * r);
     const float small = ps * 0.15915494309189535f;
-    const float big = sn / (6.283185307179586f * hf);
-    return (q == 0.0) ? small : big;
+    const float big = (r * ps) * abt_rcpf(6.283185307179586f * (float)h);
+    return (m == 0.0) ? small : big;
 }
 
 constexpr double SERIES_QA = 1.5;    // |q_perp| * a (rad) below which the FP64 series is used
+constexpr float SERIES_QA2 = 2.25f;  // SERIES_QA^2
 constexpr int SERIES_N = 24;         // truncation error < 1e-19 relative at |q| a = 1.5
 
 // 2 * sum_n i^n h_n(a,b,c) / (n+2)!  (triangle form factor / area, vertex phases a,b,c rad)
@@ -211,12 +262,30 @@ __host__ __device__ __forceinline__ void tri_series(double a, double b, double c
 }
 
 
-__host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict__ r,
-                                               const double* __restrict__ vrecs,
-                                               const DevGrid& g, const BinIn& bi,
-                                               int fast_diff, double acc[8])
+// Edge term of the rotation-invariant form factor: c_j sinc(K_j/2) e^{i mu_j},
+// with half vertex phases ha, hb (revolutions, FP64).
+__host__ __device__ __forceinline__ void edge_term(double ha, double hb, float fqx, float fqy,
+                                                   float edx, float edy, float& ssr, float& ssi)
 {
-    // ---- FP64 geometry ----
+    unsigned s1, s2;
+    const float r = half_reduce(ha + hb, &s1);     // mu_j = (delta_j + delta_{j+1}) / 2
+    const float sc = sinc_half(hb - ha, &s2);      // (delta_{j+1} - delta_j) / 2
+    const float z = r * r;
+    const float sm = r * sin_over_r(z), cm = cos_poly(z);
+    const float t = flipsign(fmaf(fqx, edy, -fqy * edx) * sc, s1 ^ s2);
+    ssr = fmaf(t, cm, ssr);
+    ssi = fmaf(t, sm, ssi);
+}
+
+// Mixed-precision body: one beam, one bin.  acc holds S+ (acc[0..3]) and S-
+// (acc[4..7]), the sums of w r+ and w' r- over beams (see mixed_finish).
+__host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict__ r,
+                                                        const double* __restrict__ vrecs,
+                                                        const DevGrid& g, const BinIn& bi,
+                                                        int fast_diff, double acc[8])
+{
+    const float* rf = reinterpret_cast<const float*>(r + R_F32);
+    // ---- FP64 geometry: bin direction from the aperture centre, |r - com| ----
     const double dx = bi.x - r[R_COM], dy = bi.y - r[R_COM + 1], dz = bi.z - r[R_COM + 2];
     const double s = fma(dx, dx, fma(dy, dy, dz * dz));
     double y0 = (double)abt_rsqf((float)s);
@@ -224,34 +293,32 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     y0 = fma(0.5 * y0, e, y0);
     e = fma(-s * y0, y0, 1.0);
     const double invd = fma(0.5 * y0, e, y0);          // 1/|r - com|
-    const double ux = dx * invd, uy = dy * invd, uz = dz * invd;   // bin direction
+    const double ux = dx * invd, uy = dy * invd, uz = dz * invd;
     if (fast_diff) {
         if (ux * r[R_PROP0] + uy * r[R_PROP0 + 1] + uz * r[R_PROP0 + 2] < r[R_COSFOV]) return;
     }
-    const double krev = g.krev;                            // k / (2 pi)
-    const double dist = s * invd;
     const double* R = r + R_RR;
     const double k1 = fma(R[0], ux, fma(R[1], uy, R[2] * uz));
     const double k2 = fma(R[3], ux, fma(R[4], uy, R[5] * uz));
     const double k3 = fma(R[6], ux, fma(R[7], uy, R[8] * uz));
-    // q in the aperture plane, revolutions per um
-    const double qx = krev * (r[R_PROP2] - k1);
-    const double qy = krev * (r[R_PROP2 + 1] - k2);
-    float fsb, fcb;
-    sincos_rev64(krev * dist, &fsb, &fcb);               // e^{i k |r - com|}
+    // q = k (prop2 - khat) in the aperture plane, revolutions per um
+    const double qx = g.krev * (r[R_PROP2] - k1);
+    const double qy = g.krev * (r[R_PROP2 + 1] - k2);
+    unsigned sb_;
+    const float rb = half_reduce(g.krev * (s * invd), &sb_);   // e^{i k |r - com|}
+    const float fsb = flipsign(rb * sin_over_r(rb * rb), sb_);
+    const float fcb = flipsign(cos_poly(rb * rb), sb_);
     const float fqx = (float)qx, fqy = (float)qy;
     const float q2 = fmaf(fqx, fqx, fqy * fqy);
-    const double qa = 6.283185307179586 * sqrt((double)q2) * r[R_AMAX];
 
-    // ---- aperture integral: sum_ref = i F for the clockwise aperture frame ----
+    // ---- aperture integral: sum_ref (the reference's edge sum) ----
     float fr_, fi_;
-    const int nv = (int)r[R_NV];
-    const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
-    if (qa < SERIES_QA) {
+    const bool tri = rf[F_NV] == 3.0f;
+    if (q2 * rf[F_QA2] < SERIES_QA2) {
         // FP64 series of the form factor about q = 0; sum_ref = -i * (signed-area F)
         const double tw = 6.283185307179586;
         double sr = 0.0, si = 0.0;
-        if (nv == 3) {
+        if (tri) {
             const double a = tw * fma(qx, r[R_EX], qy * r[R_EY]);
             const double b = tw * fma(qx, r[R_EX + 1], qy * r[R_EY + 1]);
             const double c = tw * fma(qx, r[R_EX + 2], qy * r[R_EY + 2]);
@@ -260,6 +327,8 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
             sr = r[R_AREA] * tr;
             si = r[R_AREA] * ti;
         } else {
+            const int nv = (int)rf[F_NV];
+            const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
             for (int j = 0; j < nv; j++) {
                 const int jp = (j == nv - 1) ? 0 : j + 1;
                 const double xj = vr[VREC * j + V_X], yj = vr[VREC * j + V_Y];
@@ -277,33 +346,27 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
         fi_ = (float)(-sr);
     } else {
         float ssr = 0.f, ssi = 0.f;
-        // half vertex phases (revolutions), FP64
-        double hp_prev = 0.0, hp_first = 0.0;
-        for (int j = 0; j <= nv; j++) {
-            const int jj = (j == nv) ? 0 : j;
-            double xj, yj;
-            if (nv <= 3) { xj = r[R_EX + jj]; yj = r[R_EY + jj]; }
-            else { xj = vr[VREC * jj + V_X]; yj = vr[VREC * jj + V_Y]; }
-            const double hp = 0.5 * fma(qx, xj, qy * yj);
-            if (j == 0) { hp_first = hp; hp_prev = hp; continue; }
-            const double hcur = (j == nv) ? hp_first : hp;
-            const double mu = hp_prev + hcur;        // (delta_j + delta_{j+1}) / 2
-            const double hh = hcur - hp_prev;        // (delta_{j+1} - delta_j) / 2
-            const int je = j - 1;
-            double exj, eyj;
-            if (nv <= 3) { exj = r[R_EDX + je]; eyj = r[R_EDY + je]; }
-            else { exj = vr[VREC * je + V_DX]; eyj = vr[VREC * je + V_DY]; }
-            const float cj = fmaf(fqx, (float)eyj, -fqy * (float)exj);
-            const float sc = sinc_rev64(hh);
-            float sm, cm;
-            sincos_rev64(mu, &sm, &cm);
-            const float t = cj * sc;
-            ssr = fmaf(t, cm, ssr);
-            ssi = fmaf(t, sm, ssi);
-            hp_prev = hcur;
+        if (tri) {
+            const double h0 = fma(qx, r[R_HX], qy * r[R_HY]);
+            const double h1 = fma(qx, r[R_HX + 1], qy * r[R_HY + 1]);
+            const double h2 = fma(qx, r[R_HX + 2], qy * r[R_HY + 2]);
+            edge_term(h0, h1, fqx, fqy, rf[F_EDX], rf[F_EDY], ssr, ssi);
+            edge_term(h1, h2, fqx, fqy, rf[F_EDX + 1], rf[F_EDY + 1], ssr, ssi);
+            edge_term(h2, h0, fqx, fqy, rf[F_EDX + 2], rf[F_EDY + 2], ssr, ssi);
+        } else {
+            const int nv = (int)rf[F_NV];
+            const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
+            const double hfirst = 0.5 * fma(qx, vr[V_X], qy * vr[V_Y]);
+            double hprev = hfirst;
+            for (int j = 0; j < nv; j++) {
+                const double hn = (j == nv - 1) ? hfirst
+                                                : 0.5 * fma(qx, vr[VREC * (j + 1) + V_X], qy * vr[VREC * (j + 1) + V_Y]);
+                edge_term(hprev, hn, fqx, fqy, (float)vr[VREC * j + V_DX], (float)vr[VREC * j + V_DY], ssr, ssi);
+                hprev = hn;
+            }
         }
-        // sum_ref = -(1/2pi) * sum / |q|^2
-        const float sc = -0.15915494309189535f / q2;
+        // sum_ref = -(1/2pi) sum / |q|^2
+        const float sc = -0.15915494309189535f * abt_rcpf(q2);
         fr_ = ssr * sc;
         fi_ = ssi * sc;
     }
@@ -312,58 +375,75 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     const float fR = (fcb * fr_ - fsb * fi_) * il;
     const float fI = (fcb * fi_ + fsb * fr_) * il;
 
-    // ---- polarisation, FP32 ----
+    // ---- polarisation, FP32 (karczewski + rot4 of diff_mod.f90:124-198, 630-643) ----
     // the |k_y| > 0.999999 clamp (diff_mod.f90:151) is decided in FP64, as the reference does
     const bool kclamp = fabs(k2) > K2CLAMP;
     const float fk1 = (float)k1, fk3 = (float)k3;
     const float fk2 = kclamp ? copysignf(0.99999898672103882f, (float)k2) : (float)k2;
     const float s2 = fmaf(fk1, fk1, fk3 * fk3);                // 1 - k2^2 (unclamped)
-    const float s2c = kclamp ? 2.02655792e-06f : s2;           // clamped: 1 - 0.999999^2
+    const float s2c = kclamp ? 2.02655679e-06f : s2;           // clamped: 1 - 0.999999^2
     const float rs = abt_rsqf(s2c);
     const float sq = s2c * rs;                                   // sqrt(1 - k2^2)
-    const float sig = (float)r[R_FRAC];                          // 1/sqrt(1 - Ky^2)
-    const float Kx = (float)r[R_PROP2], Ky = (float)r[R_PROP2 + 1], Kz = (float)r[R_PROP2 + 2];
-    const float frac = sq * sig;
-    const float ifrac = rs / sig;
-    const float da = -0.5f * fmaf(Kz, frac, fk3 * ifrac);        // diff_ampl(1,1) = (2,2)
-    const float db = 0.5f * fmaf(Kx * Ky, frac, -fk1 * fk2 * ifrac);  // diff_ampl(1,2) = -(2,1)
+    const float frac = sq * rf[F_SIG];
+    const float ifrac = rs * rf[F_ISIG];
+    const float da = -0.5f * fmaf(rf[F_KZ], frac, fk3 * ifrac);          // diff_ampl(1,1) = (2,2)
+    const float db = 0.5f * fmaf(rf[F_KXY], frac, -fk1 * fk2 * ifrac);   // diff_ampl(1,2) = -(2,1)
     // m and evo2 = unit(k x m) in closed form
     const float m1 = -fk1 * fk2 * rs, m2 = sq, m3 = -fk2 * fk3 * rs;
-    const float r13 = abt_rsqf(s2);
+    const float r13 = kclamp ? abt_rsqf(s2) : rs;
     const float ev1 = -fk3 * r13, ev3 = fk1 * r13;
-    // hc = R . unit(inc x khat), inc = (0,0,-1): global (u_y, -u_x, 0)
+    // hc = rot2.rot . unit(inc x khat), inc = (0,0,-1): global (u_y, -u_x, 0);
+    // forward fallback (diff_mod.f90:630-635): the bin (same phi, theta_i)
     float hx = (float)uy, hy = (float)(-ux);
     if (!(fabs(uz) < FWD_DOT)) {
         hx = (float)(bi.yt - r[R_COM + 1]);
         hy = (float)(-(bi.xt - r[R_COM]));
     }
     const float rh = abt_rsqf(fmaf(hx, hx, hy * hy));
-    hx *= rh; hy *= rh;
-    const float h1 = fmaf((float)R[0], hx, (float)R[1] * hy);
-    const float h2 = fmaf((float)R[3], hx, (float)R[4] * hy);
-    const float h3 = fmaf((float)R[6], hx, (float)R[7] * hy);
+    hx *= rh;
+    hy *= rh;
+    const float h1 = fmaf(rf[F_RR + 0], hx, rf[F_RR + 1] * hy);
+    const float h2 = fmaf(rf[F_RR + 3], hx, rf[F_RR + 4] * hy);
+    const float h3 = fmaf(rf[F_RR + 6], hx, rf[F_RR + 7] * hy);
     const float p = fmaf(h1, m1, fmaf(h2, m2, h3 * m3));
     const float qh = fmaf(h1, ev1, h3 * ev3);
-    // rot4 . diff_ampl = gg I + hh J,  J = [[0,1],[-1,0]]
+    // rot4 . diff_ampl = gg I + hh J, J = [[0,1],[-1,0]]
     const float gg = fmaf(p, da, qh * db);
-    const float hhh = fmaf(p, db, -qh * da);
-    // weights: acc += u A + v (J A)
-    const double ur = (double)(fR * gg), ui = (double)(fI * gg);
-    const double vr_ = (double)(fR * hhh), vi = (double)(fI * hhh);
-    const double* A = r + R_AMPL;
-    // A: (1,1)=A0,A1 (2,1)=A2,A3 (1,2)=A4,A5 (2,2)=A6,A7 ; J A: (1,1)=A(2,1), (2,1)=-A(1,1),
-    // (1,2)=A(2,2), (2,2)=-A(1,2)
-#define CMAC(o, ar_, ai_, br_, bi_)                                                 \
-    acc[o] = fma(ur, ar_, fma(-ui, ai_, fma(vr_, br_, fma(-vi, bi_, acc[o]))));     \
-    acc[o + 1] = fma(ur, ai_, fma(ui, ar_, fma(vr_, bi_, fma(vi, br_, acc[o + 1]))));
-    CMAC(0, A[0], A[1], A[2], A[3])
-    CMAC(2, A[2], A[3], -A[0], -A[1])
-    CMAC(4, A[4], A[5], A[6], A[7])
-    CMAC(6, A[6], A[7], -A[4], -A[5])
-#undef CMAC
+    const float hh = fmaf(p, db, -qh * da);
+    // f (gg I + hh J) = w P + w' Q with the projectors P, Q of J:  w = f (gg + i hh),
+    // w' = f (gg - i hh);  P A = [r+; i r+],  Q A = [r-; -i r-]
+    const float ta = fR * gg, tb = fI * hh, tc = fR * hh, td = fI * gg;
+    const double wr = (double)(ta - tb), wi = (double)(tc + td);
+    const double vr_ = (double)(ta + tb), vi = (double)(td - tc);
+    const double* rp = r + R_RP;
+    const double* rm = r + R_RM;
+    acc[0] = fma(wr, rp[0], fma(-wi, rp[1], acc[0]));
+    acc[1] = fma(wr, rp[1], fma(wi, rp[0], acc[1]));
+    acc[2] = fma(wr, rp[2], fma(-wi, rp[3], acc[2]));
+    acc[3] = fma(wr, rp[3], fma(wi, rp[2], acc[3]));
+    acc[4] = fma(vr_, rm[0], fma(-vi, rm[1], acc[4]));
+    acc[5] = fma(vr_, rm[1], fma(vi, rm[0], acc[5]));
+    acc[6] = fma(vr_, rm[2], fma(-vi, rm[3], acc[6]));
+    acc[7] = fma(vr_, rm[3], fma(vi, rm[2], acc[7]));
 }
 
-// The mixed kernel sums sum_b f_b G_b A_b per bin; the phi-only rotation
+// From the projector sums S+ = acc[0..3], S- = acc[4..7] (columns 1, 2) to the
+// amplitude matrix X = [S+ + S-; i (S+ - S-)], then temp_rot1 on the right.
+__host__ __device__ __forceinline__ void apply_temp_rot1(double acc[8], double cp, double sp);
+
+__host__ __device__ __forceinline__ void mixed_finish(double acc[8], double cp, double sp)
+{
+    double x[8];
+    // layout of x: (1,1) (2,1) (1,2) (2,2), re/im
+    x[0] = acc[0] + acc[4];          x[1] = acc[1] + acc[5];           // X(1,1)
+    x[2] = -(acc[1] - acc[5]);       x[3] = acc[0] - acc[4];           // X(2,1) = i (S+1 - S-1)
+    x[4] = acc[2] + acc[6];          x[5] = acc[3] + acc[7];           // X(1,2)
+    x[6] = -(acc[3] - acc[7]);       x[7] = acc[2] - acc[6];           // X(2,2) = i (S+2 - S-2)
+    apply_temp_rot1(x, cp, sp);
+    for (int q = 0; q < 8; q++) acc[q] = x[q];
+}
+
+// The mixed kernel sums f_b G_b A_b over beams per bin; the phi-only rotation
 // temp_rot1 = [[-s,-c],[c,-s]] (diff_mod.f90:646-650) is applied once on the right:
 // (X T)(r,1) = -s X(r,1) + c X(r,2);  (X T)(r,2) = -c X(r,1) - s X(r,2).
 __host__ __device__ __forceinline__ void apply_temp_rot1(double acc[8], double cp, double sp)

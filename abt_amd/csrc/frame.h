@@ -173,6 +173,34 @@ __host__ __device__ inline void make_beam_record(const double* ampl, const doubl
     rec[R_AMAX] = amax;
     // signed area in the aperture frame: the frame normal is rr^T (0,0,1)
     rec[R_AREA] = 0.5 * (rr.a[2][0] * ax + rr.a[2][1] * ay + rr.a[2][2] * az);
+
+    // constants of the mixed-precision kernel
+    for (int j = 0; j < 3 && j < nv; j++) {
+        rec[R_HX + j] = 0.5 * rec[R_EX + j];
+        rec[R_HY + j] = 0.5 * rec[R_EY + j];
+    }
+    const double* A = rec + R_AMPL;   // sign-flipped amplitude matrix
+    // r+ = (A(1,:) - i A(2,:))/2 and r- = (A(1,:) + i A(2,:))/2, columns 1 and 2
+    rec[R_RP + 0] = 0.5 * (A[0] + A[3]); rec[R_RP + 1] = 0.5 * (A[1] - A[2]);
+    rec[R_RP + 2] = 0.5 * (A[4] + A[7]); rec[R_RP + 3] = 0.5 * (A[5] - A[6]);
+    rec[R_RM + 0] = 0.5 * (A[0] - A[3]); rec[R_RM + 1] = 0.5 * (A[1] + A[2]);
+    rec[R_RM + 2] = 0.5 * (A[4] - A[7]); rec[R_RM + 3] = 0.5 * (A[5] + A[6]);
+    float* rf = reinterpret_cast<float*>(rec + R_F32);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) rf[F_RR + 3 * i + j] = (float)rr.a[i][j];
+    rf[F_KX] = (float)prop2[0];
+    rf[F_KY] = (float)prop2[1];
+    rf[F_KZ] = (float)prop2[2];
+    rf[F_KXY] = (float)(prop2[0] * prop2[1]);
+    rf[F_SIG] = (float)rec[R_FRAC];
+    rf[F_ISIG] = (float)sqrt(rec[R_KY1]);
+    const double tpa = 6.283185307179586 * amax;
+    rf[F_QA2] = (float)(tpa * tpa);
+    rf[F_NV] = (float)nv;
+    for (int j = 0; j < 3 && j < nv; j++) {
+        rf[F_EDX + j] = (float)rec[R_EDX + j];
+        rf[F_EDY + j] = (float)rec[R_EDY + j];
+    }
 }
 
 }  // namespace abt

@@ -35,7 +35,27 @@ enum : int {
     R_EDX = 52,     // [3] x_{j+1} - x_j
     R_EDY = 55,     // [3] y_{j+1} - y_j
     R_V20 = 58,     // [9] com-relative global vertices (x,y,z per vertex)
-    REC = 68        // padded to a multiple of 4 doubles (32 bytes)
+    // mixed-precision kernel constants
+    R_HX = 68,      // [3] 0.5 * aperture-frame x of vertex j (half phases)
+    R_HY = 71,      // [3] 0.5 * aperture-frame y of vertex j
+    R_RP = 74,      // [4] r+ = (A(1,:) - i A(2,:)) / 2: re/im of columns 1, 2
+    R_RM = 78,      // [4] r- = (A(1,:) + i A(2,:)) / 2
+    R_F32 = 82,     // FP32 block (float index F_* below), 12 doubles
+    REC = 96        // padded to a multiple of 4 doubles (32 bytes)
+};
+
+// float offsets inside the FP32 block of a beam record
+enum : int {
+    F_RR = 0,       // [9] rot2.rot
+    F_KX = 9, F_KY = 10, F_KZ = 11,  // prop2
+    F_KXY = 12,     // prop2_x * prop2_y
+    F_SIG = 13,     // 1 / sqrt(1 - prop2_y^2)
+    F_ISIG = 14,    // sqrt(1 - prop2_y^2)
+    F_QA2 = 15,     // (2 pi a)^2, a = aperture radius about com
+    F_NV = 16,      // number of vertices
+    F_EDX = 17,     // [3] edge x_{j+1} - x_j
+    F_EDY = 20,     // [3] edge y_{j+1} - y_j
+    F_END = 23
 };
 
 // ---- vertex record (nv > 3): VREC doubles per vertex -------------------------

@@ -478,6 +478,26 @@ int abt_far_field_dev(const abt_dev_grid* g, const double* dev_recs, const doubl
     return 0;
 }
 
+int abt_far_field_incoherent_dev(const abt_dev_grid* g, const double* dev_recs,
+                                 const double* dev_vrecs, int64_t nb, int is_fast_diff,
+                                 double* dev_intensity, void* stream)
+{
+    if (!g || !dev_intensity) return fail(ABT_ERR_ARG, "abt_far_field_incoherent_dev: null argument");
+    if (nb < 0) return fail(ABT_ERR_ARG, "negative beam count");
+    int rc = ensure_device();
+    if (rc) return rc;
+    if (nb == 0) {
+        int e = abt_launch_zero(dev_intensity, g->nbins, (hipStream_t)stream);
+        if (e) return fail(ABT_ERR_HIP, "zero-fill launch failed: %s", hipGetErrorString((hipError_t)e));
+        return 0;
+    }
+    if (!dev_recs) return fail(ABT_ERR_ARG, "abt_far_field_incoherent_dev: null records");
+    int e = abt_launch_far_field(&g->d, dev_recs, dev_vrecs ? dev_vrecs : dev_recs, nb, 2, 16,
+                                 is_fast_diff, 1, dev_intensity, nullptr, (hipStream_t)stream);
+    if (e) return fail(ABT_ERR_HIP, "far-field launch failed: %s", hipGetErrorString((hipError_t)e));
+    return 0;
+}
+
 int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff, const abt_beams* beams,
                   const abt_beams* ext, int precision, double* ampl_far_beam, double* ampl_far_ext_diff)
 {

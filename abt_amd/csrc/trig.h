@@ -39,6 +39,20 @@
 #define ABT_CR3 -8.5443057786058814865e+1f
 #define ABT_CR4 5.9225039097985419551e+1f
 
+// fp32 sin(2 pi r) = r * h(r^2), cos(2 pi r) = k(r^2), |r| <= 1/4
+// (max abs error in exact arithmetic 6.1e-9 and 8.3e-10)
+#define ABT_HR0 6.2831852918370438308f
+#define ABT_HR1 -4.1341682323056890963e+1f
+#define ABT_HR2 8.1602533454522807081e+1f
+#define ABT_HR3 -7.658169219540134564e+1f
+#define ABT_HR4 3.9759611649540012305e+1f
+#define ABT_KR0 9.9999999955408172259e-1f
+#define ABT_KR1 -1.973920819234015311e+1f
+#define ABT_KR2 6.4939300053282044217e+1f
+#define ABT_KR3 -8.5451541679337708511e+1f
+#define ABT_KR4 6.010815923321459004e+1f
+#define ABT_KR5 -2.4760335289685230034e+1f
+
 #define ABT_2_OVER_PI 0.63661977236758138243
 #define ABT_PIO2_HI 1.5707963267948965580      // double nearest pi/2
 #define ABT_PIO2_LO 6.1232339957367658e-17     // pi/2 - ABT_PIO2_HI

@@ -29,6 +29,7 @@ EXPORTS = [
     "abt_vertex_record_doubles", "abt_prepare_beams_host", "abt_prepare_beams_dev",
     "abt_grid_create", "abt_grid_destroy", "abt_grid_bins", "abt_far_field_workspace",
     "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
+    "abt_far_field_incoherent_dev",
 ]
 
 
@@ -89,6 +90,7 @@ def load():
     L.abt_finalise_workspace.argtypes = [vp]
     L.abt_finalise_workspace.restype = ctypes.c_size_t
     L.abt_finalise_dev.argtypes = [vp, vp, vp, i, d, d, vp, vp, vp, vp, ctypes.c_size_t, vp]
+    L.abt_far_field_incoherent_dev.argtypes = [vp, vp, vp, i64, i, vp, vp]
     _lib = L
     return L
 
@@ -270,3 +272,10 @@ def finalise_dev(grid: DeviceGrid, ab, ae, mueller, m1d, params, scaling=False,
     _check(load().abt_finalise_dev(grid.handle, _ptr(ab), _ptr(ae), int(scaling), beam_energy_out,
                                    ext_energy_out, _ptr(mueller), _ptr(m1d), _ptr(params),
                                    _ptr(work), wb, _stream_ptr(stream)))
+
+
+def far_field_incoherent_dev(grid: DeviceGrid, recs, nb, out, vrecs=None, fast_diff=False,
+                             stream=None):
+    """Sum over beams of |amplC_b|_F^2 per bin (FP64), into out[nbins]."""
+    _check(load().abt_far_field_incoherent_dev(grid.handle, _ptr(recs), _ptr(vrecs), nb,
+                                               int(fast_diff), _ptr(out), _stream_ptr(stream)))

@@ -147,6 +147,14 @@ int abt_far_field_dev(const abt_dev_grid* g, const double* dev_recs, const doubl
                       int64_t nb, int precision, int is_fast_diff, double* dev_out,
                       void* dev_work, size_t work_bytes, void* stream);
 
+/* Conditioning diagnostic (no reference counterpart): dev_intensity[nbins] =
+ * sum over beams of the squared Frobenius norm of each beam's FP64 amplitude
+ * (the incoherent sum).  sqrt of it over |coherent sum| bounds how much the
+ * coherent sum amplifies per-beam rounding. */
+int abt_far_field_incoherent_dev(const abt_dev_grid* g, const double* dev_recs,
+                                 const double* dev_vrecs, int64_t nb, int is_fast_diff,
+                                 double* dev_intensity, void* stream);
+
 /* Device form of abt_finalise: amplitudes resident on the device (scaled by k in
  * place).  dev_mueller: 3*16*nbins (total, beam, ext, each (nphi, ntheta, 16));
  * dev_mueller_1d: 3*16*n_theta; dev_params: 9 doubles (as abt_finalise).

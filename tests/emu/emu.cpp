@@ -33,7 +33,7 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
             else
                 beam_bin_mixed(recs + b * REC, vrecs, g, bi, fast_diff, acc);
         }
-        if (prec != 0) apply_temp_rot1(acc, cphi[iphi], sphi[iphi]);
+        if (prec != 0) mixed_finish(acc, cphi[iphi], sphi[iphi]);
         for (int q = 0; q < 4; q++) {
             out[2 * (bin + (int64_t)nbins * q)] = acc[2 * q];
             out[2 * (bin + (int64_t)nbins * q) + 1] = acc[2 * q + 1];

@@ -252,3 +252,20 @@ def test_device_api_matches_host_api():
     lib.far_field_dev(g, recs, b.n, out, precision="fp64")
     torch.cuda.synchronize()
     np.testing.assert_array_equal(out.cpu().numpy(), want)
+
+
+def test_incoherent_diagnostic_matches_single_beam_sum(orc):
+    import torch
+    th, ph = _coarse()
+    b = random_triangles(12, seed=51, box=20.0)
+    x, y, z = orc.make_far_field_bins(th, ph)
+    want = 0
+    for i in range(b.n):
+        a = orc.diffraction(th, ph, x, y, z, LAM, b, i)
+        want = want + (abs(a) ** 2).sum(axis=(0, 1))
+    g = lib.DeviceGrid(th, ph, LAM)
+    recs = torch.from_numpy(lib.prepare_beams_host(b, LAM)[0]).cuda()
+    out = torch.empty(g.nbins, dtype=torch.float64, device="cuda")
+    lib.far_field_incoherent_dev(g, recs, b.n, out)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(out.cpu().numpy().reshape(want.shape), want, rtol=1e-8)
