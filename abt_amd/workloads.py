@@ -199,7 +199,7 @@ def _fresnel(cos_i, m):
     return rs, rp, ts, tp, cos_t
 
 
-def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0):
+def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0, return_fi=False):
     """Beam sets shaped like the tracer's output for a convex particle.
 
     Incident plane wave along (0,0,-1), e-perp (1,0,0), amplitude matrix I.
@@ -209,7 +209,8 @@ def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0):
     refracted field followed by a centroid ray through the convex body: at each
     internal hit the transmitted part leaves through that facet (one outgoing
     beam, aperture = that facet) and the reflected part continues, `rec` times.
-    Returns (beams, ext, geometry verts rotated)."""
+    Returns (beams, ext, geometry verts rotated), plus the 1-based facet ids of
+    the beams and of the ext-diff beams when return_fi is set."""
     rng = np.random.default_rng(seed)
     V = verts @ rot.T
     F = faces
@@ -225,9 +226,12 @@ def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0):
     ztop = V[:, 2].max()
     ext_list, beam_list = [], []
 
+    beam_fi, ext_fi = [], []
+
     def rec_entry(i, amp, vk, prop):
         tri = P[i]
         beam_list.append((amp, vk, prop, tri))
+        beam_fi.append(i + 1)
 
     for i in np.nonzero(ill)[0]:
         n = nrm[i]
@@ -237,6 +241,7 @@ def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0):
         R = np.array([[c, s], [-s, c]])
         amp = R * phase
         ext_list.append((amp, vk, inc.copy(), P[i]))
+        ext_fi.append(i + 1)
         cos_i = -n @ inc
         rs, rp, ts, tp, cos_t = _fresnel(cos_i, m)
         refl = inc - 2 * (inc @ n) * n
@@ -279,6 +284,9 @@ def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0):
         tri = np.array([t for *_, t in lst])
         return BeamSet.from_triangles(a8, vk, pr, np.full(len(lst), np.pi / 2), tri)
 
+    if return_fi:
+        return (pack(beam_list), pack(ext_list), V, np.array(beam_fi, dtype=np.int64),
+                np.array(ext_fi, dtype=np.int64))
     return pack(beam_list), pack(ext_list), V
 
 

@@ -65,6 +65,7 @@ def main():
     beams = stress_beams(args.beams)
     log(f"workload: {beams.n} apertures, grid {len(th)} x {len(ph)} ({time.time() - t0:.1f} s to generate)")
     g = lib.DeviceGrid(th, ph, lam)
+    g.waveno = 2 * lib.PI_R4 / lam
     nb, nbins = beams.n, g.nbins
     raw = {k: torch.from_numpy(getattr(beams, k)).to(dev) for k in
            ("ampl", "vk7", "prop", "fov", "vert_off", "verts")}
@@ -124,9 +125,16 @@ def main():
                    "precision": args.precision, "step": "device frame set-up + far-field sum + Mueller epilogue"},
     }
     ach = F_ALG * nb * nbins / (kms * 1e-3)
+    kname = "k_far_field<%d,16>" % (1 if args.precision == "mixed" else 0)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath) and nb == 200_000:
+        # HBM bytes per launch of this kernel on this workload, from the committed
+        # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel calibration
+        traffic = json.load(open(tpath)).get(kname, {}).get("bytes_per_launch")
     out["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK[args.precision] / 1e12,
-                       "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": None,
-                       "kernel": "k_far_field<%d,16>" % (1 if args.precision == "mixed" else 0),
+                       "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": traffic,
+                       "kernel": kname,
                        "kernel_ms": kms, "flop_per_beam_bin": F_ALG}
 
     if args.fp64_steps > 0 and args.precision == "mixed":
@@ -143,7 +151,8 @@ def main():
         inc = torch.empty(nbins, dtype=torch.float64, device=dev)
         lib.far_field_incoherent_dev(g, recs, nb, inc, stream=stream)
         torch.cuda.synchronize()
-        out["accuracy"] = accuracy(am, a64, mm, m64, nbins, inc.cpu().numpy())
+        # the epilogue scaled the amplitudes by k in place; the incoherent sum is unscaled
+        out["accuracy"] = accuracy(am, a64, mm, m64, nbins, inc.cpu().numpy() * g.waveno ** 2)
 
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(beams, th, ph, lam, args.cpu_sample)
