@@ -199,109 +199,198 @@ def _fresnel(cos_i, m):
     return rs, rp, ts, tp, cos_t
 
 
-def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0, return_fi=False):
-    """Beam sets shaped like the tracer's output for a convex particle.
+def _fresnel_t(cos_i, m):
+    """Fresnel coefficients (torch, complex128) for a real cosine and complex index m."""
+    import torch
+    ci = cos_i.to(torch.complex128)
+    sin_i2 = torch.clamp(1 - cos_i ** 2, min=0).to(torch.complex128)
+    cos_t = torch.sqrt(1 - sin_i2 / m ** 2)
+    rs = (ci - m * cos_t) / (ci + m * cos_t)
+    rp = (m * ci - cos_t) / (cos_t + m * ci)
+    ts = 2 * ci / (ci + m * cos_t)
+    tp = 2 * ci / (cos_t + m * ci)
+    return rs, rp, ts, tp, cos_t
 
-    Incident plane wave along (0,0,-1), e-perp (1,0,0), amplitude matrix I.
-    ext-diff beams: one per illuminated facet, prop (0,0,-1), vk7 = unit(n x prop),
-    amplitude = basis rotation times the path phase (as at beam_loop_mod.f90:502-608).
-    Outgoing beams: the external reflection from each illuminated facet, then the
-    refracted field followed by a centroid ray through the convex body: at each
-    internal hit the transmitted part leaves through that facet (one outgoing
-    beam, aperture = that facet) and the reflected part continues, `rec` times.
-    Returns (beams, ext, geometry verts rotated), plus the 1-based facet ids of
-    the beams and of the ext-diff beams when return_fi is set."""
-    rng = np.random.default_rng(seed)
-    V = verts @ rot.T
-    F = faces
-    P = V[F]                                  # (nf, 3, 3)
+
+def trace_orientations(verts, faces, rots, la=0.532, m=1.31 + 0j, rec=8, device="cpu",
+                       max_elems=1 << 25):
+    """Beam sets shaped like the tracer's output, for many orientations of one particle.
+
+    Incident plane wave along (0,0,-1) in the lab frame, e-perp (1,0,0), amplitude
+    matrix I.  For orientation o the particle is rotated by rots[o] (lab = R . particle);
+    the trace runs in the particle frame (incidence -R[2,:]) so every ray of every
+    orientation tests the same facet planes, and outputs are rotated to the lab frame.
+      ext-diff beams: one per illuminated facet, prop (0,0,-1), vk7 = unit(n x prop),
+        amplitude = basis rotation times the path phase from the top of the particle
+        (as at beam_loop_mod.f90:502-608);
+      outgoing beams: per illuminated facet the external reflection, then the refracted
+        centroid ray through the body: at each internal hit (the nearest facet plane
+        ahead whose triangle contains the hit point) the transmitted part leaves
+        through that facet (one outgoing beam, aperture = that facet) unless totally
+        reflected, and the reflected part continues, `rec` times.
+    Beams are ordered by (orientation, entry facet, depth), ext beams by (orientation,
+    facet).  The reference's near-field tracer is upstream of the diffraction path and
+    is not re-implemented; this reproduces the shape of its output (beam counts,
+    apertures, directions, amplitudes).
+
+    Returns a dict of torch tensors on `device`: "beams" and "ext" (each with ampl
+    (n,8), vk7, prop, fov, vert_off, verts, fi (1-based facet ids), orient), and
+    "beam_off", "ext_off" (norient+1 offsets)."""
+    import torch
+    dt = torch.float64
+    dev = torch.device(device)
+    V = torch.as_tensor(np.asarray(verts), dtype=dt, device=dev)
+    F = torch.as_tensor(np.asarray(faces), dtype=torch.int64, device=dev)
+    R = torch.as_tensor(np.asarray(rots), dtype=dt, device=dev).reshape(-1, 3, 3)
+    O = R.shape[0]
+    unit = lambda v: v / torch.linalg.norm(v, dim=-1, keepdim=True)
+    dot = lambda a, b: (a * b).sum(-1)
+    P = V[F]
     cen = P.mean(1)
-    nrm = _unit(np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]))
-    # outward orientation: the particle is centred at the origin and convex-ish
-    flip = np.einsum("ij,ij->i", nrm, cen - V.mean(0)) < 0
+    nrm = unit(torch.linalg.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]))
+    flip = dot(nrm, cen - V.mean(0)) < 0
     nrm[flip] *= -1
+    d0 = dot(P[:, 0], nrm)
+    nf = F.shape[0]
+    inc = -R[:, 2, :]
+    ex, ey = R[:, 0, :], R[:, 1, :]
     k = 2 * np.pi / la
-    inc = np.array([0.0, 0, -1.0])
-    ill = nrm @ inc < -1e-9
-    ztop = V[:, 2].max()
-    ext_list, beam_list = [], []
+    ztop = (-(V @ inc.T)).max(0).values
+    oi, fi = torch.nonzero((inc @ nrm.T) < -1e-9, as_tuple=True)
+    nr = oi.shape[0]
+    n = nrm[fi]
+    ic = inc[oi]
+    vk = unit(torch.linalg.cross(n, ic, dim=-1))
+    phase = torch.exp(1j * k * (ztop[oi] + dot(cen[fi], ic)))
+    c, s = dot(vk, ex[oi]), dot(vk, ey[oi])
+    amp = torch.stack([torch.stack([c, s], -1), torch.stack([-s, c], -1)], 1).to(torch.complex128)
+    amp = amp * phase[:, None, None]
+    cos_i = -dot(n, ic)
+    rs, rp, ts, tp, cos_t = _fresnel_t(cos_i, m)
+    refl = ic - 2 * dot(ic, n)[:, None] * n
+    ray = torch.arange(nr, device=dev)
+    keys, fis, amps, vks, props = [ray * (rec + 2)], [fi], [amp * torch.stack([rp, rs], 1)[:, :, None]], \
+        [vk], [unit(refl)]
+    eta = 1 / m.real
+    tdir = unit(eta * ic + (eta * cos_i - cos_t.real)[:, None] * n)
+    field = amp * torch.stack([tp, ts], 1)[:, :, None]
+    pos = cen[fi].clone()
+    vkr = vk
+    blk = max(1, max_elems // max(nf, 1))
+    for depth in range(rec):
+        nl = ray.shape[0]
+        if nl == 0:
+            break
+        j = torch.empty(nl, dtype=torch.int64, device=dev)
+        tj = torch.empty(nl, dtype=dt, device=dev)
+        for b0 in range(0, nl, blk):
+            td, ps = tdir[b0:b0 + blk], pos[b0:b0 + blk]
+            denom = td @ nrm.T
+            ok = denom > 1e-12
+            tt = torch.where(ok, (d0[None, :] - ps @ nrm.T) / torch.where(ok, denom, torch.ones_like(denom)),
+                             torch.full_like(denom, float("inf")))
+            tt = torch.where(tt < 1e-9, torch.full_like(tt, float("inf")), tt)
+            tmin, jj = tt.min(1)
+            # coplanar facets share a plane distance: take the lowest-index facet whose
+            # triangle contains the hit point (the nearest plane's if none does)
+            cand = (tt <= tmin[:, None] * (1 + 1e-9) + 1e-9) & torch.isfinite(tmin)[:, None]
+            ri, fj = torch.nonzero(cand, as_tuple=True)
+            hit = ps[ri] + tmin[ri, None] * td[ri]
+            a_, b_, c_ = P[fj, 0], P[fj, 1], P[fj, 2]
+            nn = nrm[fj]
+            inside = ((dot(torch.linalg.cross(b_ - a_, hit - a_, dim=-1), nn) >= -1e-9) &
+                      (dot(torch.linalg.cross(c_ - b_, hit - b_, dim=-1), nn) >= -1e-9) &
+                      (dot(torch.linalg.cross(a_ - c_, hit - c_, dim=-1), nn) >= -1e-9))
+            first = torch.full((td.shape[0],), nf, dtype=torch.int64, device=dev)
+            first.scatter_reduce_(0, ri[inside], fj[inside], reduce="amin")
+            jj = torch.where(first < nf, first, jj)
+            j[b0:b0 + blk] = jj
+            tj[b0:b0 + blk] = tt.gather(1, jj[:, None])[:, 0]
+        live = torch.isfinite(tj)
+        ray, j, tj, tdir, field, pos, vkr = (x[live] for x in (ray, j, tj, tdir, field, pos, vkr))
+        pos = pos + tj[:, None] * tdir
+        n2 = nrm[j]
+        ci = dot(tdir, n2)
+        rs2, rp2, ts2, tp2, ct2 = _fresnel_t(ci, 1 / m)
+        vk2 = torch.linalg.cross(n2, tdir, dim=-1)
+        nv2 = torch.linalg.norm(vk2, dim=-1, keepdim=True)
+        vk2 = torch.where(nv2 > 1e-12, vk2 / torch.where(nv2 > 1e-12, nv2, torch.ones_like(nv2)), vkr)
+        field = field * torch.exp(1j * k * m * tj)[:, None, None]
+        exits = (ct2.imag.abs() < 1e-12) & (ct2.real > 0)
+        out_dir = unit(m.real * tdir + (ct2.real - m.real * ci)[:, None] * n2)
+        keys.append(ray[exits] * (rec + 2) + depth + 1)
+        fis.append(j[exits])
+        amps.append((field * torch.stack([tp2, ts2], 1)[:, :, None])[exits])
+        vks.append(vk2[exits])
+        props.append(out_dir[exits])
+        field = field * torch.stack([rp2, rs2], 1)[:, :, None]
+        tdir = unit(tdir - 2 * ci[:, None] * n2)
+    key = torch.cat(keys)
+    order = torch.sort(key, stable=True).indices
+    bray = torch.div(key[order], rec + 2, rounding_mode="floor")
 
-    beam_fi, ext_fi = [], []
+    def pack(A, vkp, pp, facet, orient):
+        nb = facet.shape[0]
+        Ro = R[orient]
+        a8 = torch.stack([A[:, 0, 0].real, A[:, 0, 0].imag, A[:, 1, 0].real, A[:, 1, 0].imag,
+                          A[:, 0, 1].real, A[:, 0, 1].imag, A[:, 1, 1].real, A[:, 1, 1].imag], 1)
+        off = torch.zeros(O + 1, dtype=torch.int64, device=dev)
+        off[1:] = torch.cumsum(torch.bincount(orient, minlength=O), 0)
+        return {"ampl": a8.contiguous(),
+                "vk7": torch.einsum("bij,bj->bi", Ro, vkp).contiguous(),
+                "prop": torch.einsum("bij,bj->bi", Ro, pp).contiguous(),
+                "fov": torch.full((nb,), np.pi / 2, dtype=dt, device=dev),
+                "vert_off": torch.arange(nb + 1, dtype=torch.int64, device=dev) * 3,
+                "verts": torch.einsum("bij,bvj->bvi", Ro, P[facet]).reshape(-1, 3).contiguous(),
+                "fi": facet + 1, "orient": orient}, off
 
-    def rec_entry(i, amp, vk, prop):
-        tri = P[i]
-        beam_list.append((amp, vk, prop, tri))
-        beam_fi.append(i + 1)
+    beams, boff = pack(torch.cat(amps)[order], torch.cat(vks)[order], torch.cat(props)[order],
+                       torch.cat(fis)[order], oi[bray])
+    ext, eoff = pack(amp, vk, ic, fi, oi)
+    return {"beams": beams, "ext": ext, "beam_off": boff, "ext_off": eoff}
 
-    for i in np.nonzero(ill)[0]:
-        n = nrm[i]
-        vk = _unit(np.cross(n, inc))
-        phase = np.exp(1j * k * (ztop - cen[i, 2]))
-        c, s = vk[0], vk[1]  # rotation from incident e-perp (1,0,0) to vk about z
-        R = np.array([[c, s], [-s, c]])
-        amp = R * phase
-        ext_list.append((amp, vk, inc.copy(), P[i]))
-        ext_fi.append(i + 1)
-        cos_i = -n @ inc
-        rs, rp, ts, tp, cos_t = _fresnel(cos_i, m)
-        refl = inc - 2 * (inc @ n) * n
-        rec_entry(i, np.diag([rp, rs]) @ amp, vk, _unit(refl))
-        # refracted direction
-        eta = 1 / m.real
-        tdir = _unit(eta * inc + (eta * cos_i - np.real(cos_t)) * n)
-        field = np.diag([tp, ts]) @ amp
-        pos = cen[i]
-        for depth in range(rec):
-            # exit facet: first plane hit along tdir from pos (convex hull)
-            denom = nrm @ tdir
-            tt = np.einsum("ij,ij->i", P[:, 0] - pos, nrm) / np.where(denom > 1e-12, denom, np.nan)
-            tt[~(denom > 1e-12)] = np.inf
-            tt[tt < 1e-9] = np.inf
-            j = int(np.argmin(tt))
-            if not np.isfinite(tt[j]):
-                break
-            pos = pos + tt[j] * tdir
-            n2 = nrm[j]
-            ci = tdir @ n2
-            rs2, rp2, ts2, tp2, ct2 = _fresnel(ci, 1 / m)
-            vk2 = np.cross(n2, tdir)
-            vk2 = _unit(vk2) if np.linalg.norm(vk2) > 1e-12 else vk
-            field = field * np.exp(1j * k * m * tt[j])
-            if np.iscomplexobj(ct2) and abs(np.imag(ct2)) < 1e-12 and np.real(ct2) > 0:
-                out_dir = _unit(m.real * tdir + (np.real(ct2) - m.real * ci) * n2)
-                rec_entry(j, np.diag([tp2, ts2]) @ field, vk2, out_dir)
-            field = np.diag([rp2, rs2]) @ field
-            tdir = _unit(tdir - 2 * (tdir @ n2) * n2)
 
-    def pack(lst):
-        if not lst:
-            return BeamSet.empty()
-        amp = np.array([a for a, *_ in lst])  # (nb, 2, 2) complex, [r, c]
-        a8 = np.stack([amp[:, 0, 0].real, amp[:, 0, 0].imag, amp[:, 1, 0].real, amp[:, 1, 0].imag,
-                       amp[:, 0, 1].real, amp[:, 0, 1].imag, amp[:, 1, 1].real, amp[:, 1, 1].imag], 1)
-        vk = np.array([v for _, v, _, _ in lst])
-        pr = np.array([p for _, _, p, _ in lst])
-        tri = np.array([t for *_, t in lst])
-        return BeamSet.from_triangles(a8, vk, pr, np.full(len(lst), np.pi / 2), tri)
+def to_beamset(d) -> BeamSet:
+    """A dict of tensors from trace_orientations (one tree) as a host BeamSet."""
+    g = lambda x: x.detach().cpu().numpy()
+    return BeamSet(g(d["ampl"]), g(d["vk7"]), g(d["prop"]), g(d["fov"]), g(d["vert_off"]), g(d["verts"]))
 
+
+def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0, return_fi=False):
+    """One orientation of trace_orientations as host beam sets.
+    Returns (beams, ext, geometry verts rotated to the lab frame), plus the 1-based
+    facet ids of the beams and of the ext-diff beams when return_fi is set."""
+    t = trace_orientations(verts, faces, np.asarray(rot)[None], la=la, m=m, rec=rec)
+    V = np.asarray(verts) @ np.asarray(rot).T
+    b, e = to_beamset(t["beams"]), to_beamset(t["ext"])
     if return_fi:
-        return (pack(beam_list), pack(ext_list), V, np.array(beam_fi, dtype=np.int64),
-                np.array(ext_fi, dtype=np.int64))
-    return pack(beam_list), pack(ext_list), V
+        return b, e, V, t["beams"]["fi"].numpy(), t["ext"]["fi"].numpy()
+    return b, e, V
 
 
-def column_orientations(n, seed=0, radius=10.0, length=20.0, nfhr=8, nfpl=14, sigma=0.5,
-                        rec=10, la=0.532):
-    """Beam sets of `n` orientations drawn uniformly on SO(3) (seeded) of one rough column."""
+def random_rotations(n, seed):
+    """n rotation matrices uniform on SO(3) (unit quaternions from a seeded normal)."""
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+                     np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+                     np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+# particles of the multi-orientation workloads: (radius, length, nfhr, nfpl, sigma, m, rec)
+PARTICLES = {
+    "rough_column": (10.0, 20.0, 8, 14, 0.5, 1.31 + 0j, 10),     # 2112 facets
+    "large_column": (50.0, 100.0, 24, 48, 0.0, 1.31 + 0.01j, 12),  # 20736 facets
+}
+
+
+def orientation_workload(norient, particle="rough_column", seed=0, la=0.532, device="cpu"):
+    """Beams of `norient` orientations (uniform on SO(3), seeded) of a PARTICLES entry."""
+    radius, length, nfhr, nfpl, sigma, m, rec = PARTICLES[particle]
     verts, faces = hex_column(radius, length, nfhr, nfpl, sigma, seed=seed)
-    rng = np.random.default_rng(seed + 1)
-    out = []
-    for _ in range(n):
-        qv = _unit(rng.normal(size=4))
-        w, x, y, z = qv
-        R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
-                      [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
-                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
-        b, e, _ = column_beams(verts, faces, R, la=la, rec=rec)
-        out.append((b, e))
-    return out
+    rots = random_rotations(norient, seed + 1)
+    t = trace_orientations(verts, faces, rots, la=la, m=m, rec=rec, device=device)
+    t["verts0"], t["faces"], t["rots"] = verts, faces, rots
+    return t
