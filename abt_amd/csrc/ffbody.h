@@ -113,17 +113,7 @@ __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict_
     double sb, cb;
     sincos_f64(bvk, &sb, &cb);
     double ar = 0.0, ai = 0.0;
-    const int nv = (int)r[R_NV];
-    const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
-    for (int j = 0; j < nv; j++) {
-        double xj, yj, mj, nj, dx, dy;
-        if (nv <= 3) {
-            xj = r[R_EX + j]; yj = r[R_EY + j]; mj = r[R_EM + j]; nj = r[R_EN + j];
-            dx = r[R_EDX + j]; dy = r[R_EDY + j];
-        } else {
-            const double* v = vr + VREC * j;
-            xj = v[V_X]; yj = v[V_Y]; mj = v[V_M]; nj = v[V_N]; dx = v[V_DX]; dy = v[V_DY];
-        }
+    const auto edge = [&](double xj, double yj, double mj, double nj, double dx, double dy) {
         const double delta = kxx * xj + kyy * yj;
         const double delta1 = kyy * mj + kxx;
         const double delta2 = kxx * nj + kyy;
@@ -139,6 +129,19 @@ __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict_
         const double sumre = -(alpha * (s0 - s1)) + beta * (s0 - s2);
         ar = ar + (cb * sumre - sb * sumim) / lam;
         ai = ai + (cb * sumim + sb * sumre) / lam;
+    };
+    // triangles: edges inline in the record; polygons: the vertex records.  Two
+    // loops, so each reads one address space (records staged in LDS, vrecs global).
+    const int nv = (int)r[R_NV];
+    if (nv <= 3) {
+        for (int j = 0; j < nv; j++)
+            edge(r[R_EX + j], r[R_EY + j], r[R_EM + j], r[R_EN + j], r[R_EDX + j], r[R_EDY + j]);
+    } else {
+        const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
+        for (int j = 0; j < nv; j++) {
+            const double* v = vr + VREC * j;
+            edge(v[V_X], v[V_Y], v[V_M], v[V_N], v[V_DX], v[V_DY]);
+        }
     }
     const double o0 = c11r * ar - c11i * ai, o1 = c11r * ai + c11i * ar;
     const double o2 = c21r * ar - c21i * ai, o3 = c21r * ai + c21i * ar;

@@ -13,7 +13,9 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "mueller.h"
 #include "records.h"
+#include "simpne.h"
 
 namespace abt {
 
@@ -39,41 +41,18 @@ __global__ __launch_bounds__(MB) void k_scale_sum(double* __restrict__ ab, doubl
     tot[i] = b + e;
 }
 
-struct Cx { double r, i; };
-__device__ __forceinline__ Cx mc(Cx a, Cx b)  // a * conjg(b)
-{
-#pragma clang fp contract(off)
-    return Cx{a.r * b.r + a.i * b.i, a.i * b.r - a.r * b.i};
-}
-
 __device__ __forceinline__ void mueller_bin(const double* __restrict__ a, int64_t b, int64_t nb,
                                             double* __restrict__ M)
 {
-#pragma clang fp contract(off)
-    const Cx s11{a[2 * b], a[2 * b + 1]};
-    const Cx s21{a[2 * (b + nb)], a[2 * (b + nb) + 1]};
-    const Cx s12{a[2 * (b + 2 * nb)], a[2 * (b + 2 * nb) + 1]};
-    const Cx s22{a[2 * (b + 3 * nb)], a[2 * (b + 3 * nb) + 1]};
-    const Cx a11 = mc(s11, s11), a12 = mc(s12, s12), a21 = mc(s21, s21), a22 = mc(s22, s22);
-    const Cx p1112 = mc(s11, s12), p2221 = mc(s22, s21), p1121 = mc(s11, s21), p2212 = mc(s22, s12);
-    const Cx p1122 = mc(s11, s22), p1221 = mc(s12, s21), p2112 = mc(s21, s12);
-    const Cx p2111 = mc(s21, s11), p2211 = mc(s22, s11);
-    M[b + nb * 0] = 0.5 * (((a11.r + a12.r) + a21.r) + a22.r);
-    M[b + nb * 1] = 0.5 * (((a11.r - a12.r) + a21.r) - a22.r);
-    M[b + nb * 2] = p1112.r + p2221.r;
-    M[b + nb * 3] = p1112.i - p2221.i;
-    M[b + nb * 4] = 0.5 * (((a11.r + a12.r) - a21.r) - a22.r);
-    M[b + nb * 5] = 0.5 * (((a11.r - a12.r) - a21.r) + a22.r);
-    M[b + nb * 6] = p1112.r - p2221.r;
-    M[b + nb * 7] = p1112.i + p2221.i;
-    M[b + nb * 8] = p1121.r + p2212.r;
-    M[b + nb * 9] = p1121.r - p2212.r;
-    M[b + nb * 10] = p1122.r + p1221.r;
-    M[b + nb * 11] = p1122.i + p2112.i;
-    M[b + nb * 12] = p2111.i + p2212.i;
-    M[b + nb * 13] = p2111.i - p2212.i;
-    M[b + nb * 14] = p2211.i - p1221.i;
-    M[b + nb * 15] = p2211.r - p1221.r;
+    double s[8], m[16];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        s[2 * q] = a[2 * (b + nb * q)];
+        s[2 * q + 1] = a[2 * (b + nb * q) + 1];
+    }
+    mueller16(s, m);
+#pragma unroll
+    for (int e = 0; e < 16; e++) M[b + nb * e] = m[e];
 }
 
 // three amplitude arrays -> three Mueller arrays (any output may be null)
@@ -89,42 +68,10 @@ __global__ __launch_bounds__(MB) void k_mueller3(const double* __restrict__ a0,
     if (M2) mueller_bin(a2, b, nb, M2);
 }
 
-// simpne (misc_submod.f90:2392-2531) on y[k*stride], k = 0..n-1
-__device__ double simpne_dev(int n, const double* __restrict__ x, const double* __restrict__ y,
-                             int64_t stride)
+__device__ __forceinline__ double simpne_dev(int n, const double* __restrict__ x,
+                                             const double* __restrict__ y, int64_t stride)
 {
-#pragma clang fp contract(off)
-    double result = 0.0;
-    if (n <= 2) return __builtin_nan("");
-    int m = 1;
-    for (;;) {
-        const double x1 = x[m - 1], x2 = x[m], x3 = x[m + 1];
-        const double e = x3 * x3 - x1 * x1;
-        const double f = x3 * x3 * x3 - x1 * x1 * x1;
-        const double fe = x3 - x1;
-        const double del[3] = {x3 - x2, x1 - x3, x2 - x1};
-        const double g[3] = {x2 + x3, x1 + x3, x1 + x2};
-        const double p[3] = {x2 * x3, x1 * x3, x1 * x2};
-        double sum1 = 0.0;
-        for (int i = 0; i < 3; i++)
-            sum1 = sum1 + y[(int64_t)(m - 1 + i) * stride] * del[i] * (f / 3.0 - g[i] * 0.5 * e + p[i] * fe);
-        result = result - sum1 / (del[0] * del[1] * del[2]);
-        m = m + 2;
-        if (n <= m + 1) break;
-    }
-    if (n % 2 != 0) return result;
-    m = n - 2;
-    const double x3 = x[n - 1], x2 = x[n - 2], x1 = x[n - 3];
-    const double e = x3 * x3 - x2 * x2;
-    const double f = x3 * x3 * x3 - x2 * x2 * x2;
-    const double fe = x3 - x2;
-    const double del[3] = {x3 - x2, x1 - x3, x2 - x1};
-    const double g[3] = {x2 + x3, x1 + x3, x1 + x2};
-    const double p[3] = {x2 * x3, x1 * x3, x1 * x2};
-    double sum1 = 0.0;
-    for (int i = 0; i < 3; i++)
-        sum1 = sum1 + y[(int64_t)(m - 1 + i) * stride] * del[i] * (f / 3.0 - g[i] * 0.5 * e + p[i] * fe);
-    return result - sum1 / (del[0] * del[1] * del[2]);
+    return simpne_eval(n, x, y, stride);
 }
 
 // mueller_1d(theta, e) of up to three matrices: one lane per (matrix, element, theta)

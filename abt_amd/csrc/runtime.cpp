@@ -21,6 +21,7 @@
 #include "../../include/abt_diff.h"
 #include "frame.h"
 #include "records.h"
+#include "rt_internal.h"
 
 extern "C" int abt_launch_far_field(const abt::DevGrid* g, const double* recs, const double* vrecs,
                                     int64_t nb, int prec, int tb, int fast_diff, int nchunks,
@@ -40,7 +41,7 @@ using abt::DevGrid;
 // errors
 static thread_local char t_err[512];
 
-static int fail(int code, const char* fmt, ...)
+int abt_fail(int code, const char* fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -48,13 +49,8 @@ static int fail(int code, const char* fmt, ...)
     va_end(ap);
     return code;
 }
+#define fail abt_fail
 
-#define HIPCHK(expr)                                                                  \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            return fail(ABT_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));  \
-    } while (0)
 
 // ---------------------------------------------------------------------------
 // context
@@ -176,13 +172,15 @@ int tile_beams(int prec) { return prec == ABT_FP64 ? 16 : 16; }
 
 }  // namespace
 
+int abt_ensure_device(int* cus)
+{
+    int rc = ensure_device();
+    if (!rc && cus) *cus = g_ctx.cus;
+    return rc;
+}
+
 // ---------------------------------------------------------------------------
 // device grid
-struct abt_dev_grid {
-    DevGrid d;
-    void* mem = nullptr;
-    int64_t nbins = 0;
-};
 
 static void host_bins(int64_t nth, const double* th, int64_t nph, const double* ph, double* x,
                       double* y, double* z)

@@ -29,7 +29,9 @@ EXPORTS = [
     "abt_vertex_record_doubles", "abt_prepare_beams_host", "abt_prepare_beams_dev",
     "abt_grid_create", "abt_grid_destroy", "abt_grid_bins", "abt_far_field_workspace",
     "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
-    "abt_far_field_incoherent_dev",
+    "abt_far_field_incoherent_dev", "abt_batch_plan_create", "abt_batch_plan_destroy",
+    "abt_batch_workspace", "abt_batch_set_tiles", "abt_batch_get_tiles", "abt_batch_autotune",
+    "abt_batch_run", "abt_batch_capture", "abt_batch_launch",
 ]
 
 
@@ -91,6 +93,16 @@ def load():
     L.abt_finalise_workspace.restype = ctypes.c_size_t
     L.abt_finalise_dev.argtypes = [vp, vp, vp, i, d, d, vp, vp, vp, vp, ctypes.c_size_t, vp]
     L.abt_far_field_incoherent_dev.argtypes = [vp, vp, vp, i64, i, vp, vp]
+    i32 = ctypes.c_int32
+    L.abt_batch_plan_create.argtypes = [vp, i32, vp, vp, i, i, ctypes.POINTER(vp)]
+    L.abt_batch_plan_destroy.argtypes = [vp]
+    L.abt_batch_workspace.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
+    L.abt_batch_set_tiles.argtypes = [vp, i32, i32]
+    L.abt_batch_get_tiles.argtypes = [vp] + [ctypes.POINTER(i32)] * 4
+    L.abt_batch_autotune.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp]
+    L.abt_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.abt_batch_capture.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.abt_batch_launch.argtypes = [vp, vp]
     _lib = L
     return L
 
@@ -279,3 +291,89 @@ def far_field_incoherent_dev(grid: DeviceGrid, recs, nb, out, vrecs=None, fast_d
     """Sum over beams of |amplC_b|_F^2 per bin (FP64), into out[nbins]."""
     _check(load().abt_far_field_incoherent_dev(grid.handle, _ptr(recs), _ptr(vrecs), nb,
                                                int(fast_diff), _ptr(out), _stream_ptr(stream)))
+
+
+# ---------------------------------------------------------------------------
+# orientation-batched far field
+
+class BatchPlan:
+    """Orientation-averaged far field of many orientations in one launch (abt_batch_*).
+
+    beam_off / ext_off: norient+1 offsets of each orientation's records in the
+    beam-tree and ext-tree record arrays.  Outputs of run(): mueller (16*nbins,
+    summed over orientations), mueller_1d (16*ntheta, optional) and integrals
+    (6*norient: scatt, I2, scatt_beam, I2_beam, scatt_ext, I2_ext; asymmetry = I2/scatt)."""
+
+    TILE_SHAPES = [(tb, bpt) for bpt in (1, 2) for tb in (8, 16, 32)]
+
+    def __init__(self, grid: DeviceGrid, beam_off, ext_off, precision="mixed", fast_diff=False):
+        self.grid = grid
+        self.beam_off = np.ascontiguousarray(beam_off, dtype=np.int64)
+        self.ext_off = np.ascontiguousarray(ext_off, dtype=np.int64)
+        self.norient = len(self.beam_off) - 1
+        if len(self.ext_off) != self.norient + 1:
+            raise AbtError("beam_off and ext_off must have the same length")
+        h = ctypes.c_void_p()
+        _check(load().abt_batch_plan_create(grid.handle, self.norient, _ptr(self.beam_off),
+                                            _ptr(self.ext_off), PRECISIONS[precision],
+                                            int(fast_diff), ctypes.byref(h)))
+        self.handle = h
+
+    def workspace_bytes(self):
+        n = ctypes.c_size_t()
+        _check(load().abt_batch_workspace(self.handle, ctypes.byref(n)))
+        return n.value
+
+    def set_tiles(self, beams_per_tile, bins_per_thread):
+        _check(load().abt_batch_set_tiles(self.handle, beams_per_tile, bins_per_thread))
+
+    def tiles(self):
+        v = [ctypes.c_int32() for _ in range(4)]
+        _check(load().abt_batch_get_tiles(self.handle, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("beams_per_tile", "bins_per_thread", "chunks", "workgroups"),
+                        (x.value for x in v)))
+
+    def autotune(self, recs_b, recs_e, vrecs_b=None, vrecs_e=None, sample=0, stream=None):
+        """Time every tile shape on the first `sample` orientations; keep the fastest.
+        Returns {(beams_per_tile, bins_per_thread): ms}."""
+        ms = np.zeros(6)
+        _check(load().abt_batch_autotune(self.handle, _ptr(recs_b), _ptr(vrecs_b), _ptr(recs_e),
+                                         _ptr(vrecs_e), int(sample), _stream_ptr(stream), _ptr(ms)))
+        return dict(zip(self.TILE_SHAPES, ms.tolist()))
+
+    def run(self, recs_b, recs_e, mueller, integrals, mueller_1d=None, vrecs_b=None, vrecs_e=None,
+            stream=None):
+        _check(load().abt_batch_run(self.handle, _ptr(recs_b), _ptr(vrecs_b), _ptr(recs_e),
+                                    _ptr(vrecs_e), _ptr(mueller), _ptr(mueller_1d), _ptr(integrals),
+                                    _stream_ptr(stream)))
+
+    def capture(self, recs_b, recs_e, mueller, integrals, mueller_1d=None, vrecs_b=None,
+                vrecs_e=None, stream=None):
+        """Capture run() with these buffers into a HIP graph held by the plan."""
+        _check(load().abt_batch_capture(self.handle, _ptr(recs_b), _ptr(vrecs_b), _ptr(recs_e),
+                                        _ptr(vrecs_e), _ptr(mueller), _ptr(mueller_1d),
+                                        _ptr(integrals), _stream_ptr(stream)))
+
+    def launch(self, stream=None):
+        """Replay the captured graph."""
+        _check(load().abt_batch_launch(self.handle, _stream_ptr(stream)))
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            load().abt_batch_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def integrals_to_params(integrals):
+    """Per-orientation scattering cross sections and asymmetry parameters from
+    BatchPlan integrals (outputs_mod.f90:133-135, 198-200)."""
+    I = np.asarray(integrals).reshape(-1, 6)
+    return {"scatt": I[:, 0], "asymmetry": I[:, 1] / I[:, 0],
+            "scatt_beam": I[:, 2], "asymmetry_beam": I[:, 3] / I[:, 2],
+            "scatt_ext_diff": I[:, 4], "asymmetry_ext_diff": I[:, 5] / I[:, 4]}

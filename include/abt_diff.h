@@ -165,6 +165,61 @@ int abt_finalise_dev(const abt_dev_grid* g, double* dev_ampl_beam, double* dev_a
                      double* dev_mueller, double* dev_mueller_1d, double* dev_params,
                      void* dev_work, size_t work_bytes, void* stream);
 
+/* ---- orientation-batched far field ------------------------------------------
+ * Replaces the orientation loop of the reference drivers (pbt_seq_mod.f90:146-269;
+ * pbt_mpi_mod.f90 spreads it over MPI ranks): for every orientation diff_main of
+ * the beam tree and of the ext tree, finalise (x k, total = beam + ext, Mueller
+ * matrix, integrals; outputs_mod.f90:63-278) and summation (outputs_mod.f90:407-471),
+ * as one persistent kernel over all orientations.
+ *
+ * Records: the prepared records (abt_prepare_beams_*) of every orientation's
+ * beam tree back to back, and likewise for the ext trees (prepared with
+ * fov_value pi/2); orientation o owns beam records beam_off[o]..beam_off[o+1]-1
+ * and ext records ext_off[o]..ext_off[o+1]-1 (host arrays of norient+1 offsets).
+ *
+ * Outputs (device):
+ *   dev_mueller[16*nbins]     sum over orientations of the total Mueller matrix
+ *                             (mueller_total before divide_by_num_orientations);
+ *   dev_mueller_1d[16*ntheta] its phi integral (may be NULL) = the sum of the
+ *                             orientations' mueller_1d up to rounding;
+ *   dev_integrals[6*norient]  per orientation: scatt, I2, scatt_beam, I2_beam,
+ *                             scatt_ext_diff, I2_ext_diff, where asymmetry = I2 / scatt
+ *                             (outputs_mod.f90:133-135, 198-200).
+ * A plan owns its workspace: runs of one plan must not overlap.  The grid must
+ * outlive the plan.  abt_batch_run only launches kernels (no allocation, no
+ * synchronisation, no memset), so callers may capture it in a HIP graph;
+ * abt_batch_capture / abt_batch_launch do that inside the library. */
+typedef struct abt_batch_plan abt_batch_plan;
+int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t* beam_off,
+                          const int64_t* ext_off, int precision, int is_fast_diff,
+                          abt_batch_plan** out);
+int abt_batch_plan_destroy(abt_batch_plan* plan);
+/* Device bytes the plan holds (partial sums, weights, tables). */
+int abt_batch_workspace(const abt_batch_plan* plan, size_t* bytes);
+/* Tile shape: beams staged per LDS tile (8, 16, 32) and bins per lane (1, 2).
+ * Default 16 x 1 until set or autotuned. */
+int abt_batch_set_tiles(abt_batch_plan* plan, int32_t beams_per_tile, int32_t bins_per_thread);
+int abt_batch_get_tiles(const abt_batch_plan* plan, int32_t* beams_per_tile,
+                        int32_t* bins_per_thread, int32_t* chunks, int32_t* workgroups);
+/* Time every tile shape on the first sample_orient orientations (0: 32) and keep
+ * the fastest; ms_out (optional) receives the 6 timings, bins_per_thread major.
+ * Synchronises the stream. */
+int abt_batch_autotune(abt_batch_plan* plan, const double* dev_recs_beam,
+                       const double* dev_vrecs_beam, const double* dev_recs_ext,
+                       const double* dev_vrecs_ext, int32_t sample_orient, void* stream,
+                       double* ms_out);
+int abt_batch_run(const abt_batch_plan* plan, const double* dev_recs_beam,
+                  const double* dev_vrecs_beam, const double* dev_recs_ext,
+                  const double* dev_vrecs_ext, double* dev_mueller, double* dev_mueller_1d,
+                  double* dev_integrals, void* stream);
+/* Capture abt_batch_run with these buffers into a HIP graph held by the plan
+ * (stream must not be NULL), then replay it with abt_batch_launch. */
+int abt_batch_capture(abt_batch_plan* plan, const double* dev_recs_beam,
+                      const double* dev_vrecs_beam, const double* dev_recs_ext,
+                      const double* dev_vrecs_ext, double* dev_mueller, double* dev_mueller_1d,
+                      double* dev_integrals, void* stream);
+int abt_batch_launch(const abt_batch_plan* plan, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

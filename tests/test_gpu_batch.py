@@ -1,0 +1,208 @@
+"""Orientation-batched far field (abt_batch_*) against the orientation loop of the
+reference drivers (pbt_seq_mod.f90:146-269) run with the CPU FP64 oracle:
+per orientation diff_main + finalise, Mueller matrices summed (summation,
+outputs_mod.f90:407-471), per-orientation scattering cross sections and asymmetry
+parameters.
+
+Tolerances
+  FP64 batched kernel vs oracle loop: summed Mueller elements |dS_ij| / S11 <= 1e-9
+  off the exact forward directions of the beams (1e-6 there, the reference
+  formula's own cancellation, as for the single-orientation FP64 kernel);
+  integrals (scatt, asymmetry) relative 1e-9.
+  Mixed batched kernel vs FP64 batched kernel: |dS_ij| / S11 <= 1e-5 and S11
+  relative <= 1e-6 on every bin brighter than 1e-6 of the brightest.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from abt_amd import lib
+from abt_amd.beams import BeamSet, concat
+from abt_amd.workloads import grid, random_triangles, to_beamset, trace_orientations, hex_column, \
+    random_rotations
+
+pytestmark = pytest.mark.gpu
+LAM = 0.532
+EXT_FOV = O.PI_R4 / 2
+
+
+def _coarse():
+    return grid((0, 5, 180), (0, 10, 350))      # 37 x 36
+
+
+def _synthetic_orientations():
+    """Uneven beam lists, including an empty ext tree, an empty beam tree and one
+    orientation with no beams at all."""
+    nb = [37, 0, 5, 120, 1, 0]
+    ne = [3, 2, 0, 9, 7, 0]
+    bl, el = [], []
+    for o, (n1, n2) in enumerate(zip(nb, ne)):
+        b = random_triangles(n1, seed=100 + o, box=20.0) if n1 else BeamSet.empty()
+        e = random_triangles(n2, seed=200 + o, box=20.0) if n2 else BeamSet.empty()
+        e.prop[:] = np.array([0.0, 0.0, -1.0])
+        bl.append(b)
+        el.append(e)
+    return bl, el
+
+
+def _column_orientations(n=4):
+    v, f = hex_column(10.0, 20.0, 8, 14, 0.5, seed=0)
+    t = trace_orientations(v, f, random_rotations(n, 7), rec=4)
+    B, E = to_beamset(t["beams"]), to_beamset(t["ext"])
+    bo, eo = t["beam_off"].numpy(), t["ext_off"].numpy()
+    return ([B.subset(np.arange(bo[o], bo[o + 1])) for o in range(n)],
+            [E.subset(np.arange(eo[o], eo[o + 1])) for o in range(n)])
+
+
+def _oracle_loop(th, ph, bl, el):
+    """mueller_total, mueller_1d_total (summation) and per-orientation parameters."""
+    msum, m1sum, params = 0.0, 0.0, []
+    for b, e in zip(bl, el):
+        rb, re = O.diff_main(th, ph, LAM, b, e, nthreads=8, raw=True)
+        f = O.finalise(th, ph, LAM, rb, re)
+        msum = msum + f["mueller"]
+        m1sum = m1sum + f["mueller_1d"]
+        params.append([f["scatt"], f["asymmetry"], f["scatt_beam"], f["asymmetry_beam"],
+                       f["scatt_ext_diff"], f["asymmetry_ext_diff"]])
+    return msum, m1sum, np.array(params)
+
+
+class Batch:
+    """Device state of one batched case."""
+
+    def __init__(self, th, ph, bl, el, precision, fast_diff=False):
+        import torch
+        self.torch = torch
+        self.dev = torch.device("cuda:0")
+        self.g = lib.DeviceGrid(th, ph, LAM)
+        B, E = concat(bl), concat(el)
+        self.bo = np.concatenate([[0], np.cumsum([b.n for b in bl])])
+        self.eo = np.concatenate([[0], np.cumsum([e.n for e in el])])
+        rb, vb = lib.prepare_beams_host(B, LAM)
+        re, ve = lib.prepare_beams_host(E, LAM, fov_value=EXT_FOV)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(self.dev)
+        self.rb, self.vb, self.re, self.ve = t(rb), t(vb), t(re), t(ve)
+        self.plan = lib.BatchPlan(self.g, self.bo, self.eo, precision, fast_diff)
+        n = self.g.nbins
+        self.m = torch.empty(16 * n, dtype=torch.float64, device=self.dev)
+        self.m1 = torch.empty(16 * len(th), dtype=torch.float64, device=self.dev)
+        self.i = torch.empty(6 * len(bl), dtype=torch.float64, device=self.dev)
+
+    def run(self):
+        self.plan.run(self.rb, self.re, self.m, self.i, self.m1, self.vb, self.ve)
+        self.torch.cuda.synchronize()
+        return (self.m.cpu().numpy().reshape(16, -1).copy(), self.i.cpu().numpy().reshape(-1, 6).copy(),
+                self.m1.cpu().numpy().reshape(16, -1).copy())
+
+
+def _forward_bins(th, ph, bl):
+    from tests.test_gpu_parity import _forward_mask
+    return _forward_mask(th, ph, [b for b in bl if b.n]).ravel()
+
+
+def _params(I):
+    I = np.asarray(I).reshape(-1, 6)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        return np.stack([I[:, 0], I[:, 1] / I[:, 0], I[:, 2], I[:, 3] / I[:, 2], I[:, 4], I[:, 5] / I[:, 4]], 1)
+
+
+def _check_params(got, ref, tol):
+    both_nan = np.isnan(got) & np.isnan(ref)
+    d = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-300)
+    d[both_nan] = 0.0
+    # an asymmetry of a tree with zero intensity is 0/0 in both
+    assert np.nanmax(d) <= tol, (np.nanmax(d), got, ref)
+
+
+@pytest.mark.parametrize("case", ["synthetic", "column"])
+def test_fp64_batch_matches_oracle_loop(case):
+    th, ph = _coarse()
+    bl, el = _synthetic_orientations() if case == "synthetic" else _column_orientations()
+    ref_m, ref_1d, ref_p = _oracle_loop(th, ph, bl, el)
+    bt = Batch(th, ph, bl, el, "fp64")
+    m, I, m1 = bt.run()
+    s11 = np.abs(ref_m[0])
+    fwd = _forward_bins(th, ph, bl + el)
+    err = np.abs(m - ref_m).max(0) / np.maximum(s11, 1e-300)
+    assert err[~fwd].max() <= 1e-9, err[~fwd].max()
+    if fwd.any():
+        assert err[fwd].max() <= 1e-6
+    _check_params(_params(I), ref_p, 1e-9)
+    # the phi integral of the sum equals the sum of the per-orientation phi integrals
+    assert np.abs(m1 - ref_1d).max() <= 1e-9 * np.abs(ref_1d[0]).max()
+
+
+def test_mixed_batch_matches_fp64_batch():
+    th, ph = _coarse()
+    bl, el = _column_orientations()
+    m64, I64, _ = Batch(th, ph, bl, el, "fp64").run()
+    mm, Im, _ = Batch(th, ph, bl, el, "mixed").run()
+    s11 = m64[0]
+    bright = s11 > 1e-6 * s11.max()
+    norm = np.abs(mm - m64)[:, bright] / s11[bright]
+    assert norm.max() <= 1e-5, norm.max(1)
+    s11r = np.abs(mm[0] - m64[0])[bright] / s11[bright]
+    assert s11r.max() <= 1e-6, s11r.max()
+    _check_params(_params(Im), _params(I64), 1e-6)
+
+
+def test_tile_shapes_agree_and_autotune():
+    th, ph = _coarse()
+    bl, el = _synthetic_orientations()
+    bt = Batch(th, ph, bl, el, "mixed")
+    ref, Iref, _ = bt.run()
+    for tb, bpt in lib.BatchPlan.TILE_SHAPES:
+        bt.plan.set_tiles(tb, bpt)
+        assert bt.plan.tiles()["beams_per_tile"] == tb
+        m, I, _ = bt.run()
+        # same beam order per bin; only the grouping of orientations into chunks may differ
+        assert np.abs(m - ref).max() <= 1e-12 * np.abs(ref[0]).max(), (tb, bpt)
+        np.testing.assert_allclose(I, Iref, rtol=1e-12, atol=0)
+    times = bt.plan.autotune(bt.rb, bt.re, bt.vb, bt.ve, sample=3)
+    assert len(times) == 6 and all(v > 0 for v in times.values()), times
+    t = bt.plan.tiles()
+    assert (t["beams_per_tile"], t["bins_per_thread"]) in times
+    m, I, _ = bt.run()
+    assert np.abs(m - ref).max() <= 1e-12 * np.abs(ref[0]).max()
+
+
+def test_batch_deterministic_and_graph_replay():
+    import torch
+    th, ph = _coarse()
+    bl, el = _column_orientations(3)
+    bt = Batch(th, ph, bl, el, "mixed")
+    m0, I0, m10 = bt.run()
+    m1, I1, _ = bt.run()
+    assert np.array_equal(m0, m1) and np.array_equal(I0, I1)
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        bt.plan.capture(bt.rb, bt.re, bt.m, bt.i, bt.m1, bt.vb, bt.ve, stream=s)
+    for rep in range(3):
+        bt.m.fill_(np.nan)
+        bt.i.fill_(np.nan)
+        torch.cuda.synchronize()
+        bt.plan.launch(stream=s)
+        s.synchronize()
+        assert np.array_equal(bt.m.cpu().numpy().reshape(16, -1), m0), rep
+        assert np.array_equal(bt.i.cpu().numpy().reshape(-1, 6), I0), rep
+        assert np.array_equal(bt.m1.cpu().numpy().reshape(16, -1), m10), rep
+        # eager work of the program between replays
+        m, I, _ = bt.run()
+        assert np.array_equal(m, m0)
+
+
+def test_batch_plan_rejects_bad_input():
+    th, ph = _coarse()
+    g = lib.DeviceGrid(th, ph, LAM)
+    with pytest.raises(lib.AbtError):
+        lib.BatchPlan(g, [0, 5, 3], [0, 1, 2])          # decreasing offsets
+    with pytest.raises(lib.AbtError):
+        lib.BatchPlan(g, [1, 5], [0, 1])                # offsets not from 0
+    g2 = lib.DeviceGrid(th, ph[:2], LAM)
+    with pytest.raises(lib.AbtError):
+        lib.BatchPlan(g2, [0, 1], [0, 1])               # simpne needs 3 phi nodes
+    p = lib.BatchPlan(g, [0, 1], [0, 1])
+    with pytest.raises(lib.AbtError):
+        p.set_tiles(12, 1)
+    with pytest.raises(lib.AbtError):
+        p.launch()                                      # nothing captured
