@@ -22,6 +22,7 @@ struct BatchArgs {
     double* mpart;            // [nchunks][16][nbins] partial Mueller sums
     double* ipart;            // [norient][ntiles][6] partial integrals
     unsigned int* counter;    // work queue
+    double* ampl_out;         // [norient][8][nbins] k-scaled total amplitude per orientation, or null
     int nchunks, ntiles, nunits, fast_diff;
 };
 

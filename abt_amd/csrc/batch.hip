@@ -58,6 +58,8 @@ __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restri
             for (int j = 0; j < BPT; j++) {
                 if (PREC == 0)
                     beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
+                else if (PREC == 3)
+                    beam_bin_fp64<true>(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
                 else
                     beam_bin_mixed(tile + bb * REC, vrecs, a.g, bi[j], a.fast_diff, acc[j]);
             }
@@ -105,8 +107,9 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
             bi[j].x = a.g.x[bc]; bi[j].y = a.g.y[bc]; bi[j].z = a.g.z[bc];
             const int bt = iphi + a.g.nphi * a.g.theta_i;
             bi[j].xt = a.g.x[bt]; bi[j].yt = a.g.y[bt]; bi[j].zt = a.g.z[bt];
-            cp[j] = PREC == 0 ? a.g.cphi[iphi] : 0.0;   // the mixed body rotates after the sum
-            sp[j] = PREC == 0 ? a.g.sphi[iphi] : 0.0;
+            bi[j].rad = PREC == 3 ? a.g.rad[bc] : 0.0;
+            cp[j] = PREC != 1 ? a.g.cphi[iphi] : 0.0;   // the mixed body rotates after the sum
+            sp[j] = PREC != 1 ? a.g.sphi[iphi] : 0.0;
         }
         const int o0 = a.chunk_o[c], o1 = a.chunk_o[c + 1];
         double* mp = a.mpart + (int64_t)c * 16 * nbins;
@@ -130,6 +133,7 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                         const int iphi = min(bin0 + j * BB, nbins - 1) % a.g.nphi;
                         mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                     }
+                    if (PREC == 3) apply_bin_phase(acc[j], a.g.kr_rev[min(bin0 + j * BB, nbins - 1)]);
 #pragma unroll
                     for (int q = 0; q < 8; q++)
                         xbs[q][j * BB + tid] = acc[j][q] * k;   // ampl_far_beam * waveno (outputs_mod.f90:104)
@@ -144,6 +148,7 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                     const int iphi = bc % a.g.nphi;
                     mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                 }
+                if (PREC == 3) apply_bin_phase(acc[j], a.g.kr_rev[bc]);
                 double xb[8], xe[8], xt[8], m[16];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
@@ -153,6 +158,11 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                 }
                 mueller16(xt, m);
                 if (bin < nbins) {
+                    if (a.ampl_out) {
+                        double* ao = a.ampl_out + (int64_t)o * 8 * nbins + bin;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) ao[(int64_t)q * nbins] = xt[q];
+                    }
                     double* mo = mp + bin;
                     if (o == o0) {
 #pragma unroll
@@ -241,6 +251,7 @@ extern "C" const void* abt_batch_kernel(int prec, int tb, int bpt)
 #define K(P, T, B) if (prec == P && tb == T && bpt == B) return batch_kernel<P, T, B>();
     K(0, 8, 1) K(0, 16, 1) K(0, 32, 1) K(0, 8, 2) K(0, 16, 2) K(0, 32, 2)
     K(1, 8, 1) K(1, 16, 1) K(1, 32, 1) K(1, 8, 2) K(1, 16, 2) K(1, 32, 2)
+    K(3, 16, 1)
 #undef K
     return nullptr;
 }

@@ -45,9 +45,14 @@ void make_chunks(const int64_t* boff, const int64_t* eoff, int n, int nch, std::
     out.push_back(n);
 }
 
+// ABI precision -> kernel variant (ABT_FP64_ACCURATE runs the accurate-phase FP64 body)
+int kernel_prec(int precision) { return precision == ABT_FP64_ACCURATE ? 3 : precision; }
+
+bool shape_allowed(int prec, int tb, int bpt) { return prec != ABT_FP64_ACCURATE || (tb == 16 && bpt == 1); }
+
 int occupancy(int prec, int tb, int bpt, int* per_cu)
 {
-    const void* k = abt_batch_kernel(prec, tb, bpt);
+    const void* k = abt_batch_kernel(kernel_prec(prec), tb, bpt);
     if (!k) return abt_fail(ABT_ERR_ARG, "no batched kernel for beams_per_tile %d, bins_per_thread %d", tb, bpt);
     int n = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, LANES, 0));
@@ -114,6 +119,7 @@ void fill_args(const abt_batch_plan* p, const Shape& s, const int32_t* chunk, co
     a->mpart = p->d_mpart;
     a->ipart = p->d_ipart;
     a->counter = p->d_counter;
+    a->ampl_out = nullptr;
     a->nchunks = s.nchunks;
     a->ntiles = s.ntiles;
     a->nunits = s.nchunks * s.ntiles;
@@ -155,7 +161,7 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
 {
     if (!g || !beam_off || !ext_off || !out) return abt_fail(ABT_ERR_ARG, "abt_batch_plan_create: null argument");
     if (norient < 1) return abt_fail(ABT_ERR_ARG, "abt_batch_plan_create: need at least one orientation");
-    if (precision != ABT_FP64 && precision != ABT_MIXED)
+    if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_ACCURATE)
         return abt_fail(ABT_ERR_ARG, "unknown precision %d", precision);
     if (g->d.nphi < 3 || g->d.ntheta < 3)
         return abt_fail(ABT_ERR_ARG, "the phi and theta integrals (simpne) need at least 3 nodes each");
@@ -178,6 +184,7 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
     p->max_chunks = 1;
     for (int tb : TB_CAND)
         for (int bpt : BPT_CAND) {
+            if (!shape_allowed(precision, tb, bpt)) continue;
             Shape s;
             rc = shape_for(p, norient, tb, bpt, &s);
             if (rc) { delete p; return rc; }
@@ -305,6 +312,11 @@ int abt_batch_autotune(abt_batch_plan* p, const double* recs_b, const double* vr
     for (int bpt : BPT_CAND) {
         for (int tb : TB_CAND) {
             if (rc || he != hipSuccess) break;
+            if (!shape_allowed(p->prec, tb, bpt)) {
+                if (ms_out) ms_out[idx] = 0.0;
+                idx++;
+                continue;
+            }
             Shape s;
             rc = shape_for(p, ns, tb, bpt, &s);
             if (rc) break;
@@ -315,7 +327,8 @@ int abt_batch_autotune(abt_batch_plan* p, const double* recs_b, const double* vr
             float ms = 0.f;
             for (int rep = 0; rep < 2 && !rc; rep++) {   // first launch warms up
                 he = hipEventRecord(e0, st);
-                int le = abt_launch_batch(&a, p->prec, tb, bpt, s.wgs, ns, out, nullptr, out + 16 * nbins, st);
+                int le = abt_launch_batch(&a, kernel_prec(p->prec), tb, bpt, s.wgs, ns, out, nullptr,
+                                          out + 16 * nbins, st);
                 if (le) rc = abt_fail(ABT_ERR_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)le));
                 if (he == hipSuccess) he = hipEventRecord(e1, st);
                 if (he == hipSuccess) he = hipEventSynchronize(e1);
@@ -345,7 +358,7 @@ int abt_batch_autotune(abt_batch_plan* p, const double* recs_b, const double* vr
 
 int abt_batch_run(const abt_batch_plan* p, const double* recs_b, const double* vrecs_b,
                   const double* recs_e, const double* vrecs_e, double* dev_mueller,
-                  double* dev_mueller_1d, double* dev_integrals, void* stream)
+                  double* dev_mueller_1d, double* dev_integrals, double* dev_ampl, void* stream)
 {
     if (!p || !dev_mueller || !dev_integrals) return abt_fail(ABT_ERR_ARG, "abt_batch_run: null argument");
     int rc = check_recs(recs_b, recs_e, p);
@@ -354,7 +367,8 @@ int abt_batch_run(const abt_batch_plan* p, const double* recs_b, const double* v
     if (rc) return rc;
     abt::BatchArgs a;
     fill_args(p, p->shape, p->d_chunk, recs_b, vrecs_b, recs_e, vrecs_e, &a);
-    int e = abt_launch_batch(&a, p->prec, p->shape.tb, p->shape.bpt, p->shape.wgs, p->norient,
+    a.ampl_out = dev_ampl;
+    int e = abt_launch_batch(&a, kernel_prec(p->prec), p->shape.tb, p->shape.bpt, p->shape.wgs, p->norient,
                              dev_mueller, dev_mueller_1d, dev_integrals, (hipStream_t)stream);
     if (e) return abt_fail(ABT_ERR_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));
     return 0;
@@ -362,7 +376,7 @@ int abt_batch_run(const abt_batch_plan* p, const double* recs_b, const double* v
 
 int abt_batch_capture(abt_batch_plan* p, const double* recs_b, const double* vrecs_b,
                       const double* recs_e, const double* vrecs_e, double* dev_mueller,
-                      double* dev_mueller_1d, double* dev_integrals, void* stream)
+                      double* dev_mueller_1d, double* dev_integrals, double* dev_ampl, void* stream)
 {
     if (!p) return abt_fail(ABT_ERR_ARG, "abt_batch_capture: null plan");
     if (!stream) return abt_fail(ABT_ERR_ARG, "abt_batch_capture: capture needs a non-default stream");
@@ -371,7 +385,8 @@ int abt_batch_capture(abt_batch_plan* p, const double* recs_b, const double* vre
     drop_graph(p);
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    rc = abt_batch_run(p, recs_b, vrecs_b, recs_e, vrecs_e, dev_mueller, dev_mueller_1d, dev_integrals, stream);
+    rc = abt_batch_run(p, recs_b, vrecs_b, recs_e, vrecs_e, dev_mueller, dev_mueller_1d, dev_integrals,
+                       dev_ampl, stream);
     hipGraph_t graph = nullptr;
     hipError_t e = hipStreamEndCapture(st, &graph);
     if (rc) {

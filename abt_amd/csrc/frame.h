@@ -117,6 +117,7 @@ __host__ __device__ inline void make_beam_record(const double* ampl, const doubl
         rec[R_PROP0 + r] = prop0[r];
     }
     for (int q = 0; q < 8; q++) rec[R_AMPL + q] = anti ? -ampl[q] : ampl[q];
+    rec[R_SIGMA] = anti ? -1.0 : 1.0;
     rec[R_COSFOV] = cos(fov);
     rec[R_NV] = (double)nv;
     rec[R_VOFF] = (double)voff;

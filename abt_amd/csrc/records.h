@@ -41,6 +41,7 @@ enum : int {
     R_RP = 74,      // [4] r+ = (A(1,:) - i A(2,:)) / 2: re/im of columns 1, 2
     R_RM = 78,      // [4] r- = (A(1,:) + i A(2,:)) / 2
     R_F32 = 82,     // FP32 block (float index F_* below), 12 doubles
+    R_SIGMA = 94,   // -1 when the amplitude matrix was sign-flipped (anti-parallel), else +1
     REC = 96        // padded to a multiple of 4 doubles (32 bytes)
 };
 
@@ -74,7 +75,7 @@ struct DevGrid {
     const double* z;
     const double* cphi;                     // [nphi] cos(phi), glibc
     const double* sphi;                     // [nphi] sin(phi)
-    const double* rinv;                     // [nbins] 1/|x|
+    const double* rad;                      // [nbins] |x|
     const double* kr_rev;                   // [nbins] k|x|/(2 pi)
     const double* theta;                    // [ntheta] theta_vals, rad
     const double* phi;                      // [nphi] phi_vals, rad

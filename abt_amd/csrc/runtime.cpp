@@ -207,7 +207,7 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     if (rc) return rc;
     const int64_t nb = nth * nph;
     std::vector<double> h((size_t)(5 * nb + 3 * nph + nth));
-    double *x = h.data(), *y = x + nb, *z = y + nb, *rinv = z + nb, *krr = rinv + nb;
+    double *x = h.data(), *y = x + nb, *z = y + nb, *rad = z + nb, *krr = rad + nb;
     double *cp = krr + nb, *sp = cp + nph, *phv = sp + nph, *thv = phv + nph;
     memcpy(phv, ph, sizeof(double) * nph);
     memcpy(thv, th, sizeof(double) * nth);
@@ -222,7 +222,7 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     const double krev = waveno / 6.283185307179586476925;
     for (int64_t b = 0; b < nb; b++) {
         const double r = sqrt(x[b] * x[b] + y[b] * y[b] + z[b] * z[b]);
-        rinv[b] = 1.0 / r;
+        rad[b] = r;
         krr[b] = krev * r;
     }
     for (int64_t i = 0; i < nph; i++) {
@@ -251,7 +251,7 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     g->d.x = d;
     g->d.y = d + nb;
     g->d.z = d + 2 * nb;
-    g->d.rinv = d + 3 * nb;
+    g->d.rad = d + 3 * nb;
     g->d.kr_rev = d + 4 * nb;
     g->d.cphi = d + 5 * nb;
     g->d.sphi = d + 5 * nb + nph;

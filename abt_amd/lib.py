@@ -19,7 +19,9 @@ LIB_PATH = os.path.join(HERE, "libabt_diff.so")
 
 ABT_FP64 = 0
 ABT_MIXED = 1
-PRECISIONS = {"fp64": ABT_FP64, "mixed": ABT_MIXED, ABT_FP64: ABT_FP64, ABT_MIXED: ABT_MIXED}
+ABT_FP64_ACCURATE = 2
+PRECISIONS = {"fp64": ABT_FP64, "mixed": ABT_MIXED, "fp64_accurate": ABT_FP64_ACCURATE,
+              ABT_FP64: ABT_FP64, ABT_MIXED: ABT_MIXED, ABT_FP64_ACCURATE: ABT_FP64_ACCURATE}
 PI_R4 = float(np.float32(3.14159265358979))
 
 # every entry point declared in include/abt_diff.h
@@ -31,7 +33,8 @@ EXPORTS = [
     "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
     "abt_far_field_incoherent_dev", "abt_batch_plan_create", "abt_batch_plan_destroy",
     "abt_batch_workspace", "abt_batch_set_tiles", "abt_batch_get_tiles", "abt_batch_autotune",
-    "abt_batch_run", "abt_batch_capture", "abt_batch_launch",
+    "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_adjoint_plan_create",
+    "abt_adjoint_plan_destroy", "abt_adjoint_run",
 ]
 
 
@@ -100,9 +103,12 @@ def load():
     L.abt_batch_set_tiles.argtypes = [vp, i32, i32]
     L.abt_batch_get_tiles.argtypes = [vp] + [ctypes.POINTER(i32)] * 4
     L.abt_batch_autotune.argtypes = [vp, vp, vp, vp, vp, i32, vp, vp]
-    L.abt_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
-    L.abt_batch_capture.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.abt_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.abt_batch_capture.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.abt_batch_launch.argtypes = [vp, vp]
+    L.abt_adjoint_plan_create.argtypes = [vp, i32, vp, vp, i, ctypes.POINTER(vp)]
+    L.abt_adjoint_plan_destroy.argtypes = [vp]
+    L.abt_adjoint_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -342,17 +348,18 @@ class BatchPlan:
         return dict(zip(self.TILE_SHAPES, ms.tolist()))
 
     def run(self, recs_b, recs_e, mueller, integrals, mueller_1d=None, vrecs_b=None, vrecs_e=None,
-            stream=None):
+            ampl=None, stream=None):
+        """ampl (optional, 8*nbins*norient): per-orientation total amplitudes x k."""
         _check(load().abt_batch_run(self.handle, _ptr(recs_b), _ptr(vrecs_b), _ptr(recs_e),
                                     _ptr(vrecs_e), _ptr(mueller), _ptr(mueller_1d), _ptr(integrals),
-                                    _stream_ptr(stream)))
+                                    _ptr(ampl), _stream_ptr(stream)))
 
     def capture(self, recs_b, recs_e, mueller, integrals, mueller_1d=None, vrecs_b=None,
-                vrecs_e=None, stream=None):
+                vrecs_e=None, ampl=None, stream=None):
         """Capture run() with these buffers into a HIP graph held by the plan."""
         _check(load().abt_batch_capture(self.handle, _ptr(recs_b), _ptr(vrecs_b), _ptr(recs_e),
                                         _ptr(vrecs_e), _ptr(mueller), _ptr(mueller_1d),
-                                        _ptr(integrals), _stream_ptr(stream)))
+                                        _ptr(integrals), _ptr(ampl), _stream_ptr(stream)))
 
     def launch(self, stream=None):
         """Replay the captured graph."""
@@ -377,3 +384,35 @@ def integrals_to_params(integrals):
     return {"scatt": I[:, 0], "asymmetry": I[:, 1] / I[:, 0],
             "scatt_beam": I[:, 2], "asymmetry_beam": I[:, 3] / I[:, 2],
             "scatt_ext_diff": I[:, 4], "asymmetry_ext_diff": I[:, 5] / I[:, 4]}
+
+
+class AdjointPlan:
+    """Gradients of a loss of the batched forward's per-orientation amplitudes
+    (abt_adjoint_*): run(recs_b, recs_e, dldx) -> per-beam vertex (9) and amplitude (8)
+    gradients of both trees."""
+
+    def __init__(self, grid: DeviceGrid, beam_off, ext_off, precision="mixed"):
+        self.grid = grid
+        self.beam_off = np.ascontiguousarray(beam_off, dtype=np.int64)
+        self.ext_off = np.ascontiguousarray(ext_off, dtype=np.int64)
+        self.norient = len(self.beam_off) - 1
+        h = ctypes.c_void_p()
+        _check(load().abt_adjoint_plan_create(grid.handle, self.norient, _ptr(self.beam_off),
+                                              _ptr(self.ext_off), PRECISIONS[precision],
+                                              ctypes.byref(h)))
+        self.handle = h
+
+    def run(self, recs_b, recs_e, dldx, gv_b=None, ga_b=None, gv_e=None, ga_e=None, stream=None):
+        _check(load().abt_adjoint_run(self.handle, _ptr(recs_b), _ptr(recs_e), _ptr(dldx), _ptr(gv_b),
+                                      _ptr(ga_b), _ptr(gv_e), _ptr(ga_e), _stream_ptr(stream)))
+
+    def close(self):
+        if self.handle is not None and self.handle.value:
+            load().abt_adjoint_plan_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -41,7 +41,10 @@ enum abt_status {
 
 enum abt_precision {
     ABT_FP64 = 0,         /* plain FP64 kernel, the reference's formulas */
-    ABT_MIXED = 1         /* FP32 body, FP64 phases, FP64 series near the forward limit */
+    ABT_MIXED = 1,        /* FP32 body, FP64 phases, FP64 series near the forward limit */
+    ABT_FP64_ACCURATE = 2 /* batched plans only: ABT_FP64 with the far-field phase k|x - com|
+                             formed without cancellation (k(|x - com| - |x|) plus the per-bin
+                             k|x|); for finite-difference gradient checks */
 };
 
 /* One tree of outgoing beams (the flattened outbeamtype records of
@@ -184,7 +187,10 @@ int abt_finalise_dev(const abt_dev_grid* g, double* dev_ampl_beam, double* dev_a
  *                             orientations' mueller_1d up to rounding;
  *   dev_integrals[6*norient]  per orientation: scatt, I2, scatt_beam, I2_beam,
  *                             scatt_ext_diff, I2_ext_diff, where asymmetry = I2 / scatt
- *                             (outputs_mod.f90:133-135, 198-200).
+ *                             (outputs_mod.f90:133-135, 198-200);
+ *   dev_ampl[8*nbins*norient] optional (NULL: not written): per orientation the total
+ *                             amplitude k (ampl_far_beam + ampl_far_ext_diff)
+ *                             (outputs_mod.f90:104-108), Fortran layout per orientation.
  * A plan owns its workspace: runs of one plan must not overlap.  The grid must
  * outlive the plan.  abt_batch_run only launches kernels (no allocation, no
  * synchronisation, no memset), so callers may capture it in a HIP graph;
@@ -197,7 +203,7 @@ int abt_batch_plan_destroy(abt_batch_plan* plan);
 /* Device bytes the plan holds (partial sums, weights, tables). */
 int abt_batch_workspace(const abt_batch_plan* plan, size_t* bytes);
 /* Tile shape: beams staged per LDS tile (8, 16, 32) and bins per lane (1, 2).
- * Default 16 x 1 until set or autotuned. */
+ * Default 16 x 1 until set or autotuned (ABT_FP64_ACCURATE: 16 x 1 only). */
 int abt_batch_set_tiles(abt_batch_plan* plan, int32_t beams_per_tile, int32_t bins_per_thread);
 int abt_batch_get_tiles(const abt_batch_plan* plan, int32_t* beams_per_tile,
                         int32_t* bins_per_thread, int32_t* chunks, int32_t* workgroups);
@@ -211,14 +217,36 @@ int abt_batch_autotune(abt_batch_plan* plan, const double* dev_recs_beam,
 int abt_batch_run(const abt_batch_plan* plan, const double* dev_recs_beam,
                   const double* dev_vrecs_beam, const double* dev_recs_ext,
                   const double* dev_vrecs_ext, double* dev_mueller, double* dev_mueller_1d,
-                  double* dev_integrals, void* stream);
+                  double* dev_integrals, double* dev_ampl, void* stream);
 /* Capture abt_batch_run with these buffers into a HIP graph held by the plan
  * (stream must not be NULL), then replay it with abt_batch_launch. */
 int abt_batch_capture(abt_batch_plan* plan, const double* dev_recs_beam,
                       const double* dev_vrecs_beam, const double* dev_recs_ext,
                       const double* dev_vrecs_ext, double* dev_mueller, double* dev_mueller_1d,
-                      double* dev_integrals, void* stream);
+                      double* dev_integrals, double* dev_ampl, void* stream);
 int abt_batch_launch(const abt_batch_plan* plan, void* stream);
+
+/* ---- adjoint: gradients for shape and roughness retrieval ---------------------
+ * No reference counterpart (the reference has no derivatives); the forward it
+ * differentiates is the batched plan's per-orientation total amplitude
+ * X_o = k (ampl_far_beam + ampl_far_ext_diff) (dev_ampl of abt_batch_run).
+ * Given dev_dldx = dL/dX_o for a real loss L (re/im pairs, the same layout as
+ * dev_ampl: [norient][8][nbins]), abt_adjoint_run writes for every beam of both
+ * trees dL/d(aperture vertices) (9 doubles per beam: x,y,z of each vertex in the
+ * record's vertex order, global frame) and dL/d(amplitude matrix) (8 doubles per
+ * beam, re/im in the ampl layout), any output may be NULL.  Beams hold their own
+ * direction, e-perp vector and field of view fixed; the aperture frame, centre,
+ * polarisation, phases and form factor all follow the vertices.  Triangular
+ * apertures only (others get NaN).  Per-edge phase terms are recomputed, not
+ * stored.  A plan owns its workspace: runs of one plan must not overlap. */
+typedef struct abt_adjoint_plan abt_adjoint_plan;
+int abt_adjoint_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t* beam_off,
+                            const int64_t* ext_off, int precision, abt_adjoint_plan** out);
+int abt_adjoint_plan_destroy(abt_adjoint_plan* plan);
+int abt_adjoint_run(const abt_adjoint_plan* plan, const double* dev_recs_beam,
+                    const double* dev_recs_ext, const double* dev_dldx,
+                    double* dev_grad_verts_beam, double* dev_grad_ampl_beam,
+                    double* dev_grad_verts_ext, double* dev_grad_ampl_ext, void* stream);
 
 #ifdef __cplusplus
 }

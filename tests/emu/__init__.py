@@ -36,3 +36,31 @@ def far_field(prec, theta, phi, lam, recs, nb, vrecs=None, fast_diff=False):
                         ctypes.c_int(ti), ctypes.c_double(lam), P(x), P(y), P(z), P(cp), P(sp),
                         P(recs), P(vr), ctypes.c_int64(nb), ctypes.c_int(int(fast_diff)), P(out))
     return raw_to_complex(out, len(ph), len(th))
+
+
+def adjoint_bins(theta, phi, lam):
+    """Per-bin inputs of the adjoint: (nbins, 7) x, y, z, x_t, y_t, |x|, k|x|/(2 pi)."""
+    import oracle
+    th = np.ascontiguousarray(theta, np.float64)
+    ph = np.ascontiguousarray(phi, np.float64)
+    x, y, z = oracle.make_far_field_bins(th, ph)
+    ti = int(np.argmax(th > oracle.PI_R4 / 180)) if np.any(th > oracle.PI_R4 / 180) else len(th) - 1
+    nph = len(ph)
+    bt = (np.arange(len(x)) % nph) + nph * ti
+    r = np.sqrt(x * x + y * y + z * z)
+    k = 2 * oracle.PI_R4 / lam
+    return np.stack([x, y, z, x[bt], y[bt], r, k / (2 * np.pi) * r], 1)
+
+
+def adjoint(recs, bins, lam_p, lam):
+    """Adjoint bodies on the CPU: (gv (nb, 3, 3), ga (nb, 8))."""
+    recs = np.ascontiguousarray(recs, np.float64)
+    bins = np.ascontiguousarray(bins, np.float64)
+    lam_p = np.ascontiguousarray(lam_p, np.float64)
+    nb = recs.shape[0]
+    gv = np.empty(9 * nb)
+    ga = np.empty(8 * nb)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib().emu_adjoint(P(recs), ctypes.c_int64(nb), P(bins), P(lam_p), ctypes.c_int64(bins.shape[0]),
+                      ctypes.c_double(lam), P(gv), P(ga))
+    return gv.reshape(nb, 3, 3), ga.reshape(nb, 8)

@@ -40,3 +40,28 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
         }
     }
 }
+
+// Adjoint bodies (abt_amd/csrc/adjbody.h) on the CPU: per beam, the sum over bins.
+// bins: 7 doubles per bin (x, y, z, xt, yt, |x|, k|x|/2pi); lam: 8 doubles per bin
+// (Lam' = Lam T_phi^T); gv: 9 per beam, ga: 8 per beam.
+#include "../../abt_amd/csrc/adjbody.h"
+extern "C" void emu_adjoint(const double* recs, int64_t nb, const double* bins, const double* lam,
+                            int64_t nbins, double lambda, double* gv, double* ga)
+{
+    using namespace abt;
+    const double waveno = 2.0 * PI_R4 / lambda;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t b = 0; b < nb; b++) {
+        const double* r = recs + b * REC;
+        AdjBeam B;
+        adj_beam_from_record(r, B);
+        AdjAcc acc;
+        adj_zero(acc);
+        for (int64_t i = 0; i < nbins; i++) {
+            const double* bb = bins + 7 * i;
+            AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
+            adj_bin_fp64(B, bn, lam + 8 * i, waveno, lambda, acc);
+        }
+        adj_finish(B, r + R_V20, r[R_SIGMA], acc, gv + 9 * b, ga + 8 * b);
+    }
+}
