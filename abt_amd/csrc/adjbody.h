@@ -34,6 +34,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "ffbody.h"
 #include "records.h"
 #include "trig.h"
 
@@ -54,6 +55,10 @@ struct AdjBeam {
     double xl[3], yl[3];        // in-plane coordinates of v_l - c
     double area;                // |N| / 2
     double qa2;                 // (aperture radius)^2: series branch when |q|^2 qa2 < SERIES_QA^2
+    // FP32 copies for the mixed-precision body
+    float fex[3], fey[3], fez[3], fP[3], fKz, fC[8];
+    float fxl[3], fyl[3];
+    float fso;                  // orientation sign of (xl, yl) for the edge form
 };
 
 struct AdjAcc {
@@ -80,6 +85,18 @@ __host__ __device__ inline void adj_beam_from_record(const double* __restrict__ 
     for (int q = 0; q < 8; q++) B.C[q] = r[R_AMPL + q];
     B.area = fabs(r[R_AREA]);
     B.qa2 = r[R_AMAX] * r[R_AMAX];
+    for (int i = 0; i < 3; i++) {
+        B.fex[i] = (float)B.ex[i];
+        B.fey[i] = (float)B.ey[i];
+        B.fez[i] = (float)B.ez[i];
+        B.fP[i] = (float)B.P[i];
+        B.fxl[i] = (float)B.xl[i];
+        B.fyl[i] = (float)B.yl[i];
+    }
+    B.fKz = (float)B.Kz;
+    for (int q = 0; q < 8; q++) B.fC[q] = (float)B.C[q];
+    const double a2 = (B.xl[1] - B.xl[0]) * (B.yl[2] - B.yl[0]) - (B.xl[2] - B.xl[0]) * (B.yl[1] - B.yl[0]);
+    B.fso = a2 < 0.0 ? 1.0f : -1.0f;
 }
 
 __host__ __device__ inline void adj_zero(AdjAcc& a)
@@ -350,12 +367,294 @@ __host__ __device__ inline void adj_bin_fp64(const AdjBeam& B, const AdjBin& bn,
     for (int i = 0; i < 3; i++) acc.cb[i] += -(kb[i] - kk * kh[i]) * ir + ph * kh[i];
 }
 
-// Mixed-precision adjoint body (FP32 body, FP64 phases and accumulators); for now the
-// FP64 body.
+// sin(x)/x and its derivative for |x| < 0.5 (FP32 series)
+__host__ __device__ __forceinline__ float sinc_ser(float x)
+{
+    const float z = x * x;
+    return fmaf(z, fmaf(z, fmaf(z, -1.98412698e-4f, 8.33333333e-3f), -0.166666667f), 1.0f);
+}
+__host__ __device__ __forceinline__ float dsinc_ser(float x)
+{
+    const float z = x * x;
+    return x * fmaf(z, fmaf(z, fmaf(z, 2.20458554e-5f, -1.19047619e-3f), 3.33333333e-2f), -0.333333333f);
+}
+
+// Forward of the model in FP64 (the function the adjoint differentiates): adds
+// s' (g I + h J) C to acc (re/im, Fortran pair order), s' = e^{i k (|x - c| - |x|)}
+// (i / lambda) F with the phase formed without cancellation; the caller applies the
+// per-bin e^{i k |x|} and T_phi after the beam sum (model_finish).  Agrees with the
+// reference's formula to rounding and is smooth to ~1e-15, so finite differences
+// with steps of 1e-4 um resolve its derivatives.
+__host__ __device__ inline void model_bin_fp64(const AdjBeam& B, double x, double y, double z, double xt,
+                                               double yt, double rad, double waveno, double lambda,
+                                               double acc[8])
+{
+    const double k = waveno;
+    const double d[3] = {x - B.c[0], y - B.c[1], z - B.c[2]};
+    const double rho = sqrt(dot3(d, d));
+    const double kh[3] = {d[0] / rho, d[1] / rho, d[2] / rho};
+    const double rc = x * B.c[0] + y * B.c[1] + z * B.c[2];
+    const double drho = (B.cc - 2.0 * rc) / (rho + rad);
+    double Es, Ec;
+    sincos_f64(k * drho, &Es, &Ec);
+    const double q3[3] = {k * (B.P[0] - kh[0]), k * (B.P[1] - kh[1]), k * (B.P[2] - kh[2])};
+    double F[2], Gx[2], Gy[2];
+    tri_ff_grad(B, dot3(q3, B.ex), dot3(q3, B.ey), F, Gx, Gy);
+    const double k1 = dot3(kh, B.ex), k2 = dot3(kh, B.ey), k3 = dot3(kh, B.ez);
+    const double k2c = fabs(k2) > K2CLAMP ? copysign(K2CLAMP, k2) : k2;
+    const double sq = sqrt(1.0 - k2c * k2c), s13 = sqrt(1.0 - k2 * k2);
+    double hv0 = kh[1], hv1 = -kh[0];
+    if (!(fabs(kh[2]) < FWD_DOT)) {
+        hv0 = yt - B.c[1];
+        hv1 = -(xt - B.c[0]);
+    }
+    const double nh = sqrt(hv0 * hv0 + hv1 * hv1);
+    const double hc[3] = {hv0 / nh, hv1 / nh, 0.0};
+    double mvec[3], uu[3];
+    for (int i = 0; i < 3; i++) mvec[i] = (B.ey[i] - k2c * kh[i]) / sq;
+    cross3(kh, B.ey, uu);
+    const double p = dot3(hc, mvec), qh = dot3(hc, uu) / s13;
+    const double da = -0.5 * (B.Kz * sq + k3 / sq), db = -0.5 * k1 * k2c / sq;
+    const double g = p * da + qh * db, h = p * db - qh * da;
+    const double EFr = Ec * F[0] - Es * F[1], EFi = Ec * F[1] + Es * F[0];
+    const double sr = -EFi / lambda, si = EFr / lambda;
+    // (g I + h J) C: row 1 = g C1 + h C2, row 2 = g C2 - h C1
+    for (int cc = 0; cc < 2; cc++) {
+        const int i1 = 2 * (0 + 2 * cc), i2 = 2 * (1 + 2 * cc);
+        const double r1r = g * B.C[i1] + h * B.C[i2], r1i = g * B.C[i1 + 1] + h * B.C[i2 + 1];
+        const double r2r = g * B.C[i2] - h * B.C[i1], r2i = g * B.C[i2 + 1] - h * B.C[i1 + 1];
+        acc[i1] += sr * r1r - si * r1i;
+        acc[i1 + 1] += sr * r1i + si * r1r;
+        acc[i2] += sr * r2r - si * r2i;
+        acc[i2 + 1] += sr * r2i + si * r2r;
+    }
+}
+
+// after the beam sum of model_bin_fp64: x T_phi, then e^{i k |x|}
+__host__ __device__ inline void model_finish(double acc[8], double cp, double sp, double kr_rev)
+{
+    apply_temp_rot1(acc, cp, sp);
+    apply_bin_phase(acc, kr_rev);
+}
+
+// Mixed-precision adjoint body: the geometry, q and every phase in FP64 (reduced to a
+// half period before FP32 polynomials), the form factor's edge terms, the
+// polarisation and its adjoint in FP32, the accumulators in FP64.  Near q -> 0 the
+// FP64 series of tri_ff_grad.  Edge phases come from the three vertex half-phase
+// exponentials w_l = e^{i alpha_l / 2}: e^{i mu_j} = w_j w_{j+1},
+// e^{i K_j / 2} = w_{j+1} conj(w_j); sinc(K_j / 2) from its series for |K_j| < 1.
 __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn, const double* __restrict__ lam,
                                               double waveno, double lambda, AdjAcc& acc)
 {
-    adj_bin_fp64(B, bn, lam, waveno, lambda, acc);
+    const double k = waveno;
+    const double dx = bn.x - B.c[0], dy = bn.y - B.c[1], dz = bn.z - B.c[2];
+    const double s2 = fma(dx, dx, fma(dy, dy, dz * dz));
+    double y0 = (double)abt_rsqf((float)s2);
+    double e = fma(-s2 * y0, y0, 1.0);
+    y0 = fma(0.5 * y0, e, y0);
+    e = fma(-s2 * y0, y0, 1.0);
+    const double ir = fma(0.5 * y0, e, y0);              // 1 / rho
+    const double rho = s2 * ir;
+    const double kh[3] = {dx * ir, dy * ir, dz * ir};
+    // e^{i k rho}: k (rho - |x|) = k (|c|^2 - 2 x.c) / (rho + |x|), plus k |x|
+    const double rc = fma(bn.x, B.c[0], fma(bn.y, B.c[1], bn.z * B.c[2]));
+    const double den = rho + bn.rad;
+    double rd = (double)abt_rcpf((float)den);
+    rd = fma(fma(-den, rd, 1.0), rd, rd);
+    rd = fma(fma(-den, rd, 1.0), rd, rd);
+    const double drho = (B.cc - 2.0 * rc) * rd;
+    const double krev = k * 0.15915494309189535;
+    unsigned sg;
+    const float rE = half_reduce((bn.kr_rev - rint(bn.kr_rev)) + krev * drho, &sg);
+    const float Es = flipsign(rE * sin_over_r(rE * rE), sg), Ec = flipsign(cos_poly(rE * rE), sg);
+    const double q3[3] = {k * (B.P[0] - kh[0]), k * (B.P[1] - kh[1]), k * (B.P[2] - kh[2])};
+    const double qx = fma(q3[0], B.ex[0], fma(q3[1], B.ex[1], q3[2] * B.ex[2]));
+    const double qy = fma(q3[0], B.ey[0], fma(q3[1], B.ey[1], q3[2] * B.ey[2]));
+    const float fqx = (float)qx, fqy = (float)qy;
+    const float q2 = fmaf(fqx, fqx, fqy * fqy);
+    float F[2], Gx[2], Gy[2];
+    if ((double)q2 * B.qa2 < 2.25) {
+        double Fd[2], Gxd[2], Gyd[2];
+        tri_ff_grad(B, qx, qy, Fd, Gxd, Gyd);
+        F[0] = (float)Fd[0]; F[1] = (float)Fd[1];
+        Gx[0] = (float)Gxd[0]; Gx[1] = (float)Gxd[1];
+        Gy[0] = (float)Gyd[0]; Gy[1] = (float)Gyd[1];
+    } else {
+        // vertex half phases alpha_l / 2 in revolutions: (q . u_l) / (4 pi)
+        float wr[3], wi[3];
+        double al[3];
+        for (int l = 0; l < 3; l++) {
+            al[l] = fma(qx, B.xl[l], qy * B.yl[l]);
+            unsigned sl;
+            const float rl = half_reduce(al[l] * 0.07957747154594767, &sl);
+            wi[l] = flipsign(rl * sin_over_r(rl * rl), sl);
+            wr[l] = flipsign(cos_poly(rl * rl), sl);
+        }
+        float sr = 0.f, si = 0.f, gxr = 0.f, gxi = 0.f, gyr = 0.f, gyi = 0.f;
+        for (int j = 0; j < 3; j++) {
+            const int jn = j == 2 ? 0 : j + 1;
+            const float ex = B.fxl[jn] - B.fxl[j], ey = B.fyl[jn] - B.fyl[j];
+            const float mx = 0.5f * (B.fxl[j] + B.fxl[jn]), my = 0.5f * (B.fyl[j] + B.fyl[jn]);
+            const float x = (float)(0.5 * (al[jn] - al[j]));          // K_j / 2
+            // e^{i x} = w_{j+1} conj(w_j), e^{i mu} = w_j w_{j+1}
+            const float cx = wr[jn] * wr[j] + wi[jn] * wi[j];
+            const float sx = wi[jn] * wr[j] - wr[jn] * wi[j];
+            const float Er = wr[j] * wr[jn] - wi[j] * wi[jn];
+            const float Ei = wr[j] * wi[jn] + wi[j] * wr[jn];
+            const bool small = fabsf(x) < 0.5f;
+            const float ix = abt_rcpf(x);
+            const float Sq = sx * ix;
+            const float S = small ? sinc_ser(x) : Sq;
+            const float dS = 0.5f * (small ? dsinc_ser(x) : (cx - Sq) * ix);
+            const float cj = B.fso * fmaf(fqx, ey, -fqy * ex);
+            const float t = cj * S;
+            sr = fmaf(t, Er, sr);
+            si = fmaf(t, Ei, si);
+            const float ax = fmaf(B.fso * ey, S, cj * dS * ex), ay = fmaf(-B.fso * ex, S, cj * dS * ey);
+            gxr += ax * Er - t * mx * Ei;
+            gxi += ax * Ei + t * mx * Er;
+            gyr += ay * Er - t * my * Ei;
+            gyi += ay * Ei + t * my * Er;
+        }
+        const float iq2 = abt_rcpf(q2);
+        F[0] = -si * iq2;
+        F[1] = sr * iq2;
+        const float tx = 2.0f * fqx * iq2, ty = 2.0f * fqy * iq2;
+        Gx[0] = -gxi * iq2 - tx * F[0];
+        Gx[1] = gxr * iq2 - tx * F[1];
+        Gy[0] = -gyi * iq2 - ty * F[0];
+        Gy[1] = gyr * iq2 - ty * F[1];
+    }
+    // polarisation (FP32; the k_y clamp decided in FP64, as the reference does)
+    const float kf[3] = {(float)kh[0], (float)kh[1], (float)kh[2]};
+    const double k2d = fma(kh[0], B.ey[0], fma(kh[1], B.ey[1], kh[2] * B.ey[2]));
+    const bool clamp = fabs(k2d) > K2CLAMP;
+    const float k1 = fmaf(kf[0], B.fex[0], fmaf(kf[1], B.fex[1], kf[2] * B.fex[2]));
+    const float k3 = fmaf(kf[0], B.fez[0], fmaf(kf[1], B.fez[1], kf[2] * B.fez[2]));
+    const float k2 = (float)k2d;
+    const float k2c = clamp ? copysignf(0.99999898672103882f, k2) : k2;
+    const float s13q = fmaf(k1, k1, k3 * k3);                    // 1 - k2^2
+    const float r13 = abt_rsqf(s13q);
+    const float s13 = s13q * r13;
+    const float sqq = clamp ? 2.02655679e-06f : s13q;
+    const float rsq = clamp ? abt_rsqf(sqq) : r13;
+    const float sq = sqq * rsq;
+    const bool fb = !(fabs(kh[2]) < FWD_DOT);
+    float hvx, hvy;
+    if (fb) {
+        hvx = (float)(bn.yt - B.c[1]);
+        hvy = (float)(-(bn.xt - B.c[0]));
+    } else {
+        hvx = kf[1];
+        hvy = -kf[0];
+    }
+    const float inh = abt_rsqf(fmaf(hvx, hvx, hvy * hvy));
+    const float hc0 = hvx * inh, hc1 = hvy * inh;
+    float mv[3], ev[3];
+    float uu[3] = {kf[1] * B.fey[2] - kf[2] * B.fey[1], kf[2] * B.fey[0] - kf[0] * B.fey[2],
+                   kf[0] * B.fey[1] - kf[1] * B.fey[0]};
+    for (int i = 0; i < 3; i++) {
+        mv[i] = (B.fey[i] - k2c * kf[i]) * rsq;
+        ev[i] = uu[i] * r13;
+    }
+    const float p = fmaf(hc0, mv[0], hc1 * mv[1]);
+    const float qh = fmaf(hc0, ev[0], hc1 * ev[1]);
+    const float da = -0.5f * fmaf(B.fKz, sq, k3 * rsq);
+    const float db = -0.5f * k1 * k2c * rsq;
+    const float g = fmaf(p, da, qh * db);
+    const float h = fmaf(p, db, -qh * da);
+    // s = E (i / lambda) F
+    const float il = (float)(1.0 / lambda);
+    const float EFr = Ec * F[0] - Es * F[1], EFi = Ec * F[1] + Es * F[0];
+    const float sr = -EFi * il, si = EFr * il;
+    float L[8];
+    for (int q = 0; q < 8; q++) L[q] = (float)lam[q];
+    float z1r = 0.f, z1i = 0.f, z2r = 0.f, z2i = 0.f;
+    for (int cc = 0; cc < 2; cc++) {
+        const int i1 = 2 * (0 + 2 * cc), i2 = 2 * (1 + 2 * cc);
+        const float l1r = L[i1], l1i = L[i1 + 1], l2r = L[i2], l2i = L[i2 + 1];
+        const float c1r = B.fC[i1], c1i = B.fC[i1 + 1], c2r = B.fC[i2], c2i = B.fC[i2 + 1];
+        z1r += l1r * c1r + l1i * c1i + l2r * c2r + l2i * c2i;
+        z1i += l1r * c1i - l1i * c1r + l2r * c2i - l2i * c2r;
+        z2r += l1r * c2r + l1i * c2i - (l2r * c1r + l2i * c1i);
+        z2i += l1r * c2i - l1i * c2r - (l2r * c1i - l2i * c1r);
+    }
+    for (int cc = 0; cc < 2; cc++) {
+        const int i1 = 2 * (0 + 2 * cc), i2 = 2 * (1 + 2 * cc);
+        const float r1r = g * L[i1] - h * L[i2], r1i = g * L[i1 + 1] - h * L[i2 + 1];
+        const float r2r = g * L[i2] + h * L[i1], r2i = g * L[i2 + 1] + h * L[i1 + 1];
+        acc.GC[i1] += (double)(sr * r1r + si * r1i);
+        acc.GC[i1 + 1] += (double)(sr * r1i - si * r1r);
+        acc.GC[i2] += (double)(sr * r2r + si * r2i);
+        acc.GC[i2 + 1] += (double)(sr * r2i - si * r2r);
+    }
+    const float gg = sr * z1r - si * z1i, gh = sr * z2r - si * z2i;
+    const float mur = g * z1r + h * z2r, mui = g * z1i + h * z2i;
+    const float mEr = mur * Ec - mui * Es, mEi = mur * Es + mui * Ec;
+    const float nur = -mEi * il, nui = mEr * il;
+    const float nFr = nur * F[0] - nui * F[1], nFi = nur * F[1] + nui * F[0];
+    const float Rx = nur * Gx[0] - nui * Gx[1];
+    const float Ry = nur * Gy[0] - nui * Gy[1];
+    acc.S0 += (double)nFr;
+    const double dRx = (double)Rx, dRy = (double)Ry;
+    for (int i = 0; i < 3; i++) {
+        acc.Qx[i] = fma(q3[i], dRx, acc.Qx[i]);
+        acc.Qy[i] = fma(q3[i], dRy, acc.Qy[i]);
+    }
+    const float kw = (float)k;
+    float kb[3];
+    for (int i = 0; i < 3; i++) kb[i] = -kw * (Rx * B.fex[i] + Ry * B.fey[i]);
+    const float pb = gg * da + gh * db, qb = gg * db - gh * da;
+    const float dab = gg * p - gh * qh, dbb = gg * qh + gh * p;
+    const float Kzb = -0.5f * sq * dab;
+    float sqb = -0.5f * (B.fKz - k3 * rsq * rsq) * dab + 0.5f * k1 * k2c * rsq * rsq * dbb;
+    const float k3b = -0.5f * dab * rsq;
+    const float k1b = -0.5f * k2c * rsq * dbb;
+    float k2cb = -0.5f * k1 * rsq * dbb;
+    const float hcb0 = pb * mv[0] + qb * ev[0], hcb1 = pb * mv[1] + qb * ev[1];
+    const float mb[3] = {pb * hc0, pb * hc1, 0.f};
+    k2cb += -(mb[0] * kf[0] + mb[1] * kf[1]) * rsq;
+    sqb += -(mb[0] * mv[0] + mb[1] * mv[1]) * rsq;
+    const float ub[3] = {qb * hc0 * r13, qb * hc1 * r13, 0.f};
+    const float s13b = -qb * (hc0 * ev[0] + hc1 * ev[1]) * r13;
+    float eyb[3], t3[3];
+    for (int i = 0; i < 3; i++) {
+        eyb[i] = mb[i] * rsq;
+        kb[i] += -k2c * rsq * mb[i];
+    }
+    // kb += e_y x ub; eyb += ub x khat  (ub has no z component)
+    t3[0] = -B.fey[2] * ub[1];
+    t3[1] = B.fey[2] * ub[0];
+    t3[2] = B.fey[0] * ub[1] - B.fey[1] * ub[0];
+    for (int i = 0; i < 3; i++) kb[i] += t3[i];
+    eyb[0] += ub[1] * kf[2];
+    eyb[1] += -ub[0] * kf[2];
+    eyb[2] += ub[0] * kf[1] - ub[1] * kf[0];
+    float k2b = -k2 * r13 * s13b;
+    k2cb += -k2c * rsq * sqb;
+    if (!clamp) k2b += k2cb;
+    (void)s13;
+    for (int i = 0; i < 3; i++) {
+        kb[i] += k1b * B.fex[i] + k2b * B.fey[i] + k3b * B.fez[i];
+        acc.exb[i] += (double)(k1b * kf[i]);
+        acc.eyb[i] += (double)(eyb[i] + k2b * kf[i]);
+        acc.ezb[i] += (double)(k3b * kf[i] + Kzb * B.fP[i]);
+    }
+    const float hdot = hcb0 * hc0 + hcb1 * hc1;
+    const float hvbx = (hcb0 - hdot * hc0) * inh, hvby = (hcb1 - hdot * hc1) * inh;
+    const float tx = -hvby, ty = hvbx;
+    float cbf[3] = {0.f, 0.f, 0.f};
+    if (fb) {
+        cbf[0] = -tx;
+        cbf[1] = -ty;
+    } else {
+        kb[0] += tx;
+        kb[1] += ty;
+    }
+    const float kk = kb[0] * kf[0] + kb[1] * kf[1] + kb[2] * kf[2];
+    const float irf = (float)ir;
+    const float ph = kw * nFi;
+    for (int i = 0; i < 3; i++) acc.cb[i] += (double)(cbf[i] - (kb[i] - kk * kf[i]) * irf + ph * kf[i]);
 }
 
 // Per-beam gradients from the accumulated sums: gv[9] = dL/dv_l (x,y,z per vertex, the

@@ -31,6 +31,7 @@
 #include <math.h>
 #include <stdint.h>
 
+#include "adjbody.h"
 #include "batch.h"
 #include "ffbody.h"
 #include "mueller.h"
@@ -58,8 +59,12 @@ __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restri
             for (int j = 0; j < BPT; j++) {
                 if (PREC == 0)
                     beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
-                else if (PREC == 3)
-                    beam_bin_fp64<true>(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
+                else if (PREC == 3) {
+                    AdjBeam B;
+                    adj_beam_from_record(tile + bb * REC, B);
+                    model_bin_fp64(B, bi[j].x, bi[j].y, bi[j].z, bi[j].xt, bi[j].yt, bi[j].rad, a.g.waveno,
+                                   a.g.lambda, acc[j]);
+                }
                 else
                     beam_bin_mixed(tile + bb * REC, vrecs, a.g, bi[j], a.fast_diff, acc[j]);
             }
@@ -133,7 +138,10 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                         const int iphi = min(bin0 + j * BB, nbins - 1) % a.g.nphi;
                         mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                     }
-                    if (PREC == 3) apply_bin_phase(acc[j], a.g.kr_rev[min(bin0 + j * BB, nbins - 1)]);
+                    if (PREC == 3) {
+                        const int bc = min(bin0 + j * BB, nbins - 1);
+                        model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bc]);
+                    }
 #pragma unroll
                     for (int q = 0; q < 8; q++)
                         xbs[q][j * BB + tid] = acc[j][q] * k;   // ampl_far_beam * waveno (outputs_mod.f90:104)
@@ -148,7 +156,7 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                     const int iphi = bc % a.g.nphi;
                     mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                 }
-                if (PREC == 3) apply_bin_phase(acc[j], a.g.kr_rev[bc]);
+                if (PREC == 3) model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bc]);
                 double xb[8], xe[8], xt[8], m[16];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {

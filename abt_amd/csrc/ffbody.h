@@ -25,7 +25,7 @@ __host__ __device__ __forceinline__ float abt_rsqf(float x)
 struct BinIn {
     double x, y, z;      // bin position
     double xt, yt, zt;   // bin (same phi, theta_i): forward fallback of the hc vector
-    double rad;          // |bin position| (accurate-phase variant only)
+    double rad;          // |bin position| (FP64 model body only)
 };
 
 __host__ __device__ __forceinline__ float abt_rcpf(float x)
@@ -39,14 +39,6 @@ __host__ __device__ __forceinline__ float abt_rcpf(float x)
 
 // ---------------------------------------------------------------------------
 // FP64 plain kernel body: one beam, one bin.  acc: re/im of (1,1) (2,1) (1,2) (2,2)
-//
-// ACC (accurate phase): the far-field phase k |x - com| ~ 1.2e7 rad carries an
-// absolute rounding error of ~1e-9 rad when formed directly, as the reference does.
-// With ACC the beam's phase is k (|x - com| - |x|), with |x - com| - |x| =
-// (|com|^2 - 2 x.com) / (|x - com| + |x|) free of cancellation, and the common
-// per-bin factor e^{i k |x|} is applied by the caller after the beam sum.  Used for
-// gradient checks, where finite differences of 1e-4 um must resolve ~1e-13 rad.
-template <bool ACC = false>
 __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict__ r,
                                               const double* __restrict__ vrecs,
                                               const DevGrid& g, const BinIn& bi, double cp,
@@ -120,13 +112,7 @@ __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict_
     if (fabs(kxx) < KMIN) kxx = KMIN;
     if (fabs(kyy) < KMIN) kyy = KMIN;
     double sb, cb;
-    if (ACC) {
-        const double rc = bi.x * r[R_COM] + bi.y * r[R_COM + 1] + bi.z * r[R_COM + 2];
-        const double drho = (r[R_CC] - 2.0 * rc) / (r1 + bi.rad);
-        sincos_f64(w * drho, &sb, &cb);
-    } else {
-        sincos_f64(bvk, &sb, &cb);
-    }
+    sincos_f64(bvk, &sb, &cb);
     double ar = 0.0, ai = 0.0;
     const auto edge = [&](double xj, double yj, double mj, double nj, double dx, double dy) {
         const double delta = kxx * xj + kyy * yj;
@@ -445,7 +431,7 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     acc[7] = fma(vr_, rm[3], fma(vi, rm[2], acc[7]));
 }
 
-// The per-bin phase factor e^{i k |x|} of the accurate-phase FP64 body.
+// The per-bin phase factor e^{i k |x|} of the FP64 model body (adjbody.h).
 __host__ __device__ __forceinline__ void apply_bin_phase(double acc[8], double kr_rev)
 {
 #pragma clang fp contract(off)

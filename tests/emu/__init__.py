@@ -52,7 +52,7 @@ def adjoint_bins(theta, phi, lam):
     return np.stack([x, y, z, x[bt], y[bt], r, k / (2 * np.pi) * r], 1)
 
 
-def adjoint(recs, bins, lam_p, lam):
+def adjoint(recs, bins, lam_p, lam, mixed=False):
     """Adjoint bodies on the CPU: (gv (nb, 3, 3), ga (nb, 8))."""
     recs = np.ascontiguousarray(recs, np.float64)
     bins = np.ascontiguousarray(bins, np.float64)
@@ -61,6 +61,7 @@ def adjoint(recs, bins, lam_p, lam):
     gv = np.empty(9 * nb)
     ga = np.empty(8 * nb)
     P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
-    lib().emu_adjoint(P(recs), ctypes.c_int64(nb), P(bins), P(lam_p), ctypes.c_int64(bins.shape[0]),
-                      ctypes.c_double(lam), P(gv), P(ga))
+    f = lib().emu_adjoint_mixed if mixed else lib().emu_adjoint
+    f(P(recs), ctypes.c_int64(nb), P(bins), P(lam_p), ctypes.c_int64(bins.shape[0]), ctypes.c_double(lam),
+      P(gv), P(ga))
     return gv.reshape(nb, 3, 3), ga.reshape(nb, 8)

@@ -6,6 +6,7 @@
 #include <stdint.h>
 #include <string.h>
 #include "../../abt_amd/csrc/ffbody.h"
+#include "../../abt_amd/csrc/adjbody.h"
 
 extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, double lambda,
                               const double* x, const double* y, const double* z,
@@ -25,15 +26,22 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
     for (int bin = 0; bin < nbins; bin++) {
         const int iphi = bin % nphi;
         const int bt = iphi + nphi * theta_i;
-        BinIn bi{x[bin], y[bin], z[bin], x[bt], y[bt], z[bt]};
+        const double rad = sqrt(x[bin] * x[bin] + y[bin] * y[bin] + z[bin] * z[bin]);
+        BinIn bi{x[bin], y[bin], z[bin], x[bt], y[bt], z[bt], rad};
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int64_t b = 0; b < nb; b++) {
             if (prec == 0)
                 beam_bin_fp64(recs + b * REC, vrecs, g, bi, cphi[iphi], sphi[iphi], fast_diff, acc);
+            else if (prec == 3) {   // the FP64 model (adjbody.h)
+                AdjBeam B;
+                adj_beam_from_record(recs + b * REC, B);
+                model_bin_fp64(B, bi.x, bi.y, bi.z, bi.xt, bi.yt, rad, g.waveno, g.lambda, acc);
+            }
             else
                 beam_bin_mixed(recs + b * REC, vrecs, g, bi, fast_diff, acc);
         }
-        if (prec != 0) mixed_finish(acc, cphi[iphi], sphi[iphi]);
+        if (prec == 1) mixed_finish(acc, cphi[iphi], sphi[iphi]);
+        if (prec == 3) model_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
         for (int q = 0; q < 4; q++) {
             out[2 * (bin + (int64_t)nbins * q)] = acc[2 * q];
             out[2 * (bin + (int64_t)nbins * q) + 1] = acc[2 * q + 1];
@@ -44,7 +52,6 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
 // Adjoint bodies (abt_amd/csrc/adjbody.h) on the CPU: per beam, the sum over bins.
 // bins: 7 doubles per bin (x, y, z, xt, yt, |x|, k|x|/2pi); lam: 8 doubles per bin
 // (Lam' = Lam T_phi^T); gv: 9 per beam, ga: 8 per beam.
-#include "../../abt_amd/csrc/adjbody.h"
 extern "C" void emu_adjoint(const double* recs, int64_t nb, const double* bins, const double* lam,
                             int64_t nbins, double lambda, double* gv, double* ga)
 {
@@ -61,6 +68,28 @@ extern "C" void emu_adjoint(const double* recs, int64_t nb, const double* bins, 
             const double* bb = bins + 7 * i;
             AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
             adj_bin_fp64(B, bn, lam + 8 * i, waveno, lambda, acc);
+        }
+        adj_finish(B, r + R_V20, r[R_SIGMA], acc, gv + 9 * b, ga + 8 * b);
+    }
+}
+
+// Mixed-precision adjoint bodies on the CPU (host FP32 and 1/sqrtf for the device's).
+extern "C" void emu_adjoint_mixed(const double* recs, int64_t nb, const double* bins, const double* lam,
+                                  int64_t nbins, double lambda, double* gv, double* ga)
+{
+    using namespace abt;
+    const double waveno = 2.0 * PI_R4 / lambda;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t b = 0; b < nb; b++) {
+        const double* r = recs + b * REC;
+        AdjBeam B;
+        adj_beam_from_record(r, B);
+        AdjAcc acc;
+        adj_zero(acc);
+        for (int64_t i = 0; i < nbins; i++) {
+            const double* bb = bins + 7 * i;
+            AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
+            adj_bin_mixed(B, bn, lam + 8 * i, waveno, lambda, acc);
         }
         adj_finish(B, r + R_V20, r[R_SIGMA], acc, gv + 9 * b, ga + 8 * b);
     }
