@@ -46,6 +46,10 @@ def main():
     ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
     ap.add_argument("--fp64-steps", type=int, default=1, help="FP64 comparison steps (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=96, help="apertures in the CPU baseline sample (0: skip)")
+    ap.add_argument("--orient", type=int, default=1000,
+                    help="orientations of the batched orientation-average workload (0: skip)")
+    ap.add_argument("--adjoint-orient", type=int, default=64,
+                    help="orientations of the adjoint workload (0: skip)")
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -156,6 +160,12 @@ def main():
 
     if args.cpu_sample > 0:
         out["cpu_baseline"] = cpu_baseline(beams, th, ph, lam, args.cpu_sample)
+    del recs, amp, amp_ext, mueller, work, raw
+    torch.cuda.empty_cache()
+    if args.orient > 0:
+        out["orientation_average"] = orientation_average(args.orient, args.steps, lam, dev)
+    if args.adjoint_orient > 0:
+        out["adjoint"] = adjoint_bench(args.adjoint_orient, lam, dev)
     line = json.dumps(out)
     if args.json_out:
         with open(args.json_out, "w") as f:
@@ -189,6 +199,215 @@ def accuracy(am, a64, mm, m64, nbins, incoherent):
         sel = kappa <= kmax
         out[f"kappa_le_{int(kmax)}"] = {"bins": int(sel.sum()), "S11_rel": q(s11r[sel]),
                                         "norm_elem": q(norm[:, sel])}
+    return out
+
+
+def _prepare_trees(w, lam, dev):
+    import torch
+    from abt_amd import lib
+    R = lib.load().abt_record_doubles()
+    nb, ne = w["beams"]["ampl"].shape[0], w["ext"]["ampl"].shape[0]
+    rb = torch.empty((nb, R), dtype=torch.float64, device=dev)
+    re = torch.empty((ne, R), dtype=torch.float64, device=dev)
+    lib.prepare_beams_dev(w["beams"], nb, lam, rb, fov_value=np.pi / 2)
+    lib.prepare_beams_dev(w["ext"], ne, lam, re, fov_value=lib.PI_R4 / 2)
+    return rb, re, nb, ne
+
+
+def orientation_average(norient, steps, lam, dev):
+    """Batched orientation averaging: `norient` orientations (uniform on SO(3), seeded) of
+    a Gaussian-rough hexagonal column (radius 10 um, length 20 um, sigma 0.5 um, 2112
+    facets), 10 internal reflections, 721 theta x 180 phi bins.  The whole job (every
+    orientation's beam and ext trees, per-orientation Mueller matrices summed
+    incoherently, phi integral, per-orientation integrals) is one graph-captured
+    persistent launch with the autotuned tile shape, in mixed precision.  Beside it: the
+    plain FP64 drop-in path run as the reference drivers do (per orientation diff_main,
+    finalise, summation), and the FP64 batched kernel as the accuracy reference."""
+    import torch
+    from abt_amd import lib
+    from abt_amd.workloads import GRIDS, grid, orientation_workload
+    t0 = time.time()
+    w = orientation_workload(norient, device=dev)
+    torch.cuda.synchronize()
+    boff, eoff = w["beam_off"].cpu().numpy(), w["ext_off"].cpu().numpy()
+    counts = np.diff(boff) + np.diff(eoff)
+    th, ph = grid(*GRIDS["orient"])
+    g = lib.DeviceGrid(th, ph, lam)
+    rb, re, nb, ne = _prepare_trees(w, lam, dev)
+    del w
+    torch.cuda.synchronize()
+    nbins = g.nbins
+    bb = (nb + ne) * nbins
+    log(f"orientation average: {norient} orientations, {nb} beams + {ne} ext "
+        f"({counts.min()}-{counts.max()} per orientation), {bb:.4e} beam.bins, "
+        f"generated in {time.time() - t0:.1f} s")
+    f64 = dict(dtype=torch.float64, device=dev)
+    res = {}
+
+    def buffers():
+        return (torch.empty(16 * nbins, **f64), torch.empty(6 * norient, **f64),
+                torch.empty(16 * len(th), **f64))
+
+    # mixed precision: autotune, capture, replay
+    plan = lib.BatchPlan(g, boff, eoff, "mixed")
+    tune = plan.autotune(rb, re, sample=32)
+    tiles = plan.tiles()
+    m, I, m1 = buffers()
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        plan.capture(rb, re, m, I, m1, stream=s)
+    plan.launch(stream=s)
+    s.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        plan.launch(stream=s)
+    e1.record(s)
+    s.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    v_mixed = bb / (ms * 1e-3)
+    log(f"  mixed batched (graph): {ms:.1f} ms/step, {v_mixed:.4e} beam.bin/s, tiles {tiles}")
+    mm, Im = m.cpu().numpy().reshape(16, -1), I.cpu().numpy().reshape(-1, 6)
+    # FP64 batched (accuracy reference)
+    plan64 = lib.BatchPlan(g, boff, eoff, "fp64")
+    m64, I64, m164 = buffers()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    plan64.run(rb, re, m64, I64, m164)
+    torch.cuda.synchronize()
+    ms64b = (time.perf_counter() - t) * 1e3
+    log(f"  fp64 batched: {ms64b:.1f} ms, {bb / (ms64b * 1e-3):.4e} beam.bin/s")
+    M64, I64h = m64.cpu().numpy().reshape(16, -1), I64.cpu().numpy().reshape(-1, 6)
+    # plain FP64 drop-in path, orientation by orientation
+    ab, ae = torch.empty(8 * nbins, **f64), torch.empty(8 * nbins, **f64)
+    mu, m1d, par = torch.empty(48 * nbins, **f64), torch.empty(48 * len(th), **f64), torch.empty(16, **f64)
+    tot = torch.zeros(16 * nbins, **f64)
+    ws = max(lib.far_field_workspace(g, int(max(np.diff(boff).max(), np.diff(eoff).max())), "fp64"), 8)
+    work = torch.empty(ws // 8 + 1, **f64)
+    fwork = torch.empty(lib.load().abt_finalise_workspace(g.handle) // 8 + 1, **f64)
+
+    def plain(o):
+        b0, b1, q0, q1 = int(boff[o]), int(boff[o + 1]), int(eoff[o]), int(eoff[o + 1])
+        lib.far_field_dev(g, rb[b0:b1], b1 - b0, ab, precision="fp64", work=work)
+        lib.far_field_dev(g, re[q0:q1], q1 - q0, ae, precision="fp64", work=work)
+        lib.finalise_dev(g, ab, ae, mu, m1d, par, work=fwork)
+        tot.add_(mu[:16 * nbins])
+
+    plain(0)
+    tot.zero_()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for o in range(norient):
+        plain(o)
+    torch.cuda.synchronize()
+    dt_plain = time.perf_counter() - t
+    v_plain = bb / dt_plain
+    log(f"  fp64 plain loop: {dt_plain * 1e3:.1f} ms, {v_plain:.4e} beam.bin/s")
+    Mp = tot.cpu().numpy().reshape(16, -1)
+    s11 = np.abs(M64[0])
+    norm = np.abs(mm - M64) / s11
+    s11r = np.abs(mm[0] - M64[0]) / s11
+    q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
+    asym_m, asym_64 = Im[:, 1] / Im[:, 0], I64h[:, 1] / I64h[:, 0]
+    res = {
+        "workload": f"{norient} orientations (uniform SO(3), seed 1) of a rough hexagonal column "
+                    "(r 10 um, l 20 um, sigma 0.5 um, 2112 facets), m 1.31, 10 reflections, "
+                    "721 theta x 180 phi bins, lambda 0.532 um",
+        "beams": int(nb), "ext_beams": int(ne), "beams_per_orientation": [int(counts.min()), int(counts.max())],
+        "beam_bins": bb, "metric": "beam·bin diffractions/s", "value": v_mixed, "ms_per_step": ms,
+        "steps": steps, "launch": "one hipGraph replay per step (persistent kernel + reductions)",
+        "tiles": tiles, "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in tune.items()},
+        "fp64_plain": {"value": v_plain, "ms": dt_plain * 1e3,
+                       "path": "per orientation: abt_far_field_dev x2 + abt_finalise_dev + sum"},
+        "fp64_batched": {"value": bb / (ms64b * 1e-3), "ms": ms64b},
+        "speedup_vs_fp64_plain": v_mixed / v_plain,
+        "roofline": {"bound": "valu", "achieved": F_ALG * bb / (ms * 1e-3) / 1e12, "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": F_ALG * bb / (ms * 1e-3) / PEAK["mixed"]},
+        "accuracy": {"reference": "FP64 batched kernel on the same prepared beams",
+                     "S11_rel": q(s11r), "norm_elem": q(norm),
+                     "per_element_norm_max": [float(norm[e].max()) for e in range(16)],
+                     "plain_vs_batched_fp64_norm_max": float((np.abs(Mp - M64) / s11).max()),
+                     "asymmetry_abs_max": float(np.abs(asym_m - asym_64).max()),
+                     "scatt_rel_max": float(np.abs(Im[:, 0] / I64h[:, 0] - 1).max())},
+    }
+    for x in (plan, plan64):
+        x.close()
+    return res
+
+
+def adjoint_bench(norient, lam, dev):
+    """Adjoint workload: the rough column of the orientation average, `norient`
+    orientations, 361 theta x 90 phi bins; loss = weighted L2 of the orientation-averaged
+    S11 (relative) and -S12/S11 against the particle with vertices perturbed by
+    N(0, 0.05 um); gradients with respect to every particle vertex and every beam's
+    amplitude matrix.  Times the forward (mixed batched, per-orientation amplitudes) and
+    the backward (mixed adjoint kernel) on HIP events."""
+    import torch
+    from abt_amd import autograd as AG
+    from abt_amd import lib
+    from abt_amd.workloads import GRIDS, grid, hex_column, random_rotations, trace_orientations
+    verts, faces = hex_column(10.0, 20.0, 8, 14, 0.5, seed=0)
+    rots = random_rotations(norient, 1)
+    t = trace_orientations(verts, faces, rots, rec=10, device=dev)
+    rng = np.random.default_rng(77)
+    vt = verts + rng.normal(scale=0.05, size=verts.shape)
+    tt = trace_orientations(vt, faces, rots, rec=10, device=dev)
+    th, ph = grid(*GRIDS["adjoint"])
+    g = lib.DeviceGrid(th, ph, lam)
+    mk = lambda tr, prec: AG.FarFieldSystem(g, tr["beam_off"].cpu().numpy(), tr["ext_off"].cpu().numpy(),
+                                            {k: tr["beams"][k] for k in ("prop", "vk7")},
+                                            {k: tr["ext"][k] for k in ("prop", "vk7")}, precision=prec)
+    st = mk(tt, "mixed")
+    Xt = st.forward_amplitudes(tt["beams"]["verts"].reshape(-1, 3, 3), tt["beams"]["ampl"],
+                               tt["ext"]["verts"].reshape(-1, 3, 3), tt["ext"]["ampl"])
+    t11, t12 = AG.mueller_11_12(Xt)
+    tp = -t12 / t11
+    w = AG.solid_angle_weights(th, ph, dev)
+    facesT = torch.as_tensor(faces, device=dev)
+    R = torch.as_tensor(rots, dtype=torch.float64, device=dev)
+    nb, ne = int(t["beam_off"][-1]), int(t["ext_off"][-1])
+    out = {"workload": f"{norient} orientations of the rough column, 361 theta x 90 phi bins, "
+                       "loss: weighted L2 of orientation-averaged S11 and -S12/S11",
+           "beams": nb, "ext_beams": ne, "beam_bins": (nb + ne) * g.nbins}
+    for prec in ("mixed", "fp64"):
+        s = mk(t, prec)
+        V = torch.tensor(verts, dtype=torch.float64, device=dev, requires_grad=True)
+        ab = t["beams"]["ampl"].clone().requires_grad_(True)
+        ae = t["ext"]["ampl"].clone().requires_grad_(True)
+
+        def apert(tree):
+            tr = t[tree]
+            return torch.einsum("bij,bvj->bvi", R[tr["orient"]], V[facesT[tr["fi"] - 1]])
+
+        times = {"forward": [], "backward": []}
+        for rep in range(3):
+            V.grad = None
+            ab.grad = ae.grad = None
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+            vb, ve = apert("beams"), apert("ext")
+            torch.cuda.synchronize()
+            e[0].record()
+            X = AG.far_field(s, vb, ab, ve, ae)
+            e[1].record()
+            L = AG.retrieval_loss(X, t11, tp, w)
+            torch.cuda.synchronize()
+            e[2].record()
+            L.backward()
+            e[3].record()
+            torch.cuda.synchronize()
+            if rep:
+                times["forward"].append(e[0].elapsed_time(e[1]))
+                times["backward"].append(e[2].elapsed_time(e[3]))
+        grads = (V.grad, ab.grad, ae.grad)
+        f, b = float(np.mean(times["forward"])), float(np.mean(times["backward"]))
+        out[prec] = {"forward_ms": f, "backward_ms": b, "backward_over_forward": b / f,
+                     "loss": float(L), "grad_vertex_norm": float(grads[0].norm())}
+        log(f"  adjoint {prec}: forward {f:.1f} ms, backward {b:.1f} ms, ratio {b / f:.2f}")
+        if prec == "mixed":
+            gm = grads[0].detach().cpu().numpy()
+        else:
+            g64 = grads[0].detach().cpu().numpy()
+    out["mixed_vs_fp64_grad_rel"] = float(np.linalg.norm(gm - g64) / np.linalg.norm(g64))
     return out
 
 

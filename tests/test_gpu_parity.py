@@ -277,3 +277,21 @@ def test_incoherent_diagnostic_matches_single_beam_sum(orc):
     lib.far_field_incoherent_dev(g, recs, b.n, out)
     torch.cuda.synchronize()
     np.testing.assert_allclose(out.cpu().numpy().reshape(want.shape), want, rtol=1e-8)
+
+
+def test_fp64_and_mixed_match_golden_fixture():
+    from tests.golden import load_small
+    d, b, e = load_small()
+    th, ph, lam = d["theta"], d["phi"], float(d["lam"])
+    gb, ge = lib.diff_main(th, ph, lam, b, e, precision="fp64", raw=True)
+    for g, w in ((gb, d["ampl_far_beam"]), (ge, d["ampl_far_ext_diff"])):
+        assert np.abs(g - w).max() <= 1e-9 * np.abs(w).max()
+    fin = lib.finalise(th, ph, lam, gb, ge)
+    s11 = np.abs(d["mueller"][0])
+    assert (np.abs(fin["mueller"] - d["mueller"]) / s11.max()).max() <= 1e-9
+    mb, me = lib.diff_main(th, ph, lam, b, e, precision="mixed", raw=True)
+    fm = lib.finalise(th, ph, lam, mb, me)
+    # bright bins off the exact forward directions (where the reference formula
+    # itself cancels; the mixed kernel's series is the more accurate there)
+    ok = (s11 > 1e-6 * s11.max()) & ~_forward_mask(th, ph, [b, e]).ravel()
+    assert (np.abs(fm["mueller"] - d["mueller"])[:, ok] / s11[ok]).max() <= 1e-5
