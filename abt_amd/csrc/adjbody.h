@@ -439,8 +439,8 @@ __host__ __device__ inline void model_finish(double acc[8], double cp, double sp
 
 // Mixed-precision adjoint body: the geometry, q and every phase in FP64 (reduced to a
 // half period before FP32 polynomials), the form factor's edge terms, the
-// polarisation and its adjoint in FP32, the accumulators in FP64.  Near q -> 0 the
-// FP64 series of tri_ff_grad.  Edge phases come from the three vertex half-phase
+// polarisation and its adjoint in FP32, the accumulators in FP64.  Near q -> 0 an
+// FP32 series of the form factor and its gradient.  Edge phases come from the three vertex half-phase
 // exponentials w_l = e^{i alpha_l / 2}: e^{i mu_j} = w_j w_{j+1},
 // e^{i K_j / 2} = w_{j+1} conj(w_j); sinc(K_j / 2) from its series for |K_j| < 1.
 __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn, const double* __restrict__ lam,
@@ -474,11 +474,35 @@ __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn
     const float q2 = fmaf(fqx, fqx, fqy * fqy);
     float F[2], Gx[2], Gy[2];
     if ((double)q2 * B.qa2 < 2.25) {
-        double Fd[2], Gxd[2], Gyd[2];
-        tri_ff_grad(B, qx, qy, Fd, Gxd, Gyd);
-        F[0] = (float)Fd[0]; F[1] = (float)Fd[1];
-        Gx[0] = (float)Gxd[0]; Gx[1] = (float)Gxd[1];
-        Gy[0] = (float)Gyd[0]; Gy[1] = (float)Gyd[1];
+        // FP32 series about q = 0 (|alpha_l| <= 1.5: 14 terms leave < 1e-9)
+        float al[3];
+        for (int l = 0; l < 3; l++) al[l] = fmaf(fqx, B.fxl[l], fqy * B.fyl[l]);
+        const float e1 = al[0] + al[1] + al[2];
+        const float e2 = fmaf(al[0], al[1], fmaf(al[1], al[2], al[2] * al[0]));
+        const float e3 = al[0] * al[1] * al[2];
+        float hm3 = 0.f, hm2 = 0.f, hm1 = 1.f, hprev = 1.f, inv = 0.5f;
+        float d[3] = {0.f, 0.f, 0.f}, dr[3] = {0.f, 0.f, 0.f}, di[3] = {0.f, 0.f, 0.f};
+        float fr = 0.5f, fi = 0.f;
+        for (int n = 1; n <= 14; n++) {
+            for (int l = 0; l < 3; l++) d[l] = fmaf(al[l], d[l], hprev);
+            const float hn = fmaf(e1, hm1, fmaf(-e2, hm2, e3 * hm3));
+            inv = inv / (float)(n + 2);
+            const float t = hn * inv;
+            switch (n & 3) {
+                case 0: fr += t; for (int l = 0; l < 3; l++) dr[l] += d[l] * inv; break;
+                case 1: fi += t; for (int l = 0; l < 3; l++) di[l] += d[l] * inv; break;
+                case 2: fr -= t; for (int l = 0; l < 3; l++) dr[l] -= d[l] * inv; break;
+                default: fi -= t; for (int l = 0; l < 3; l++) di[l] -= d[l] * inv; break;
+            }
+            hm3 = hm2; hm2 = hm1; hm1 = hn; hprev = hn;
+        }
+        const float A2 = (float)(2.0 * B.area);
+        F[0] = A2 * fr;
+        F[1] = A2 * fi;
+        Gx[0] = A2 * (dr[0] * B.fxl[0] + dr[1] * B.fxl[1] + dr[2] * B.fxl[2]);
+        Gx[1] = A2 * (di[0] * B.fxl[0] + di[1] * B.fxl[1] + di[2] * B.fxl[2]);
+        Gy[0] = A2 * (dr[0] * B.fyl[0] + dr[1] * B.fyl[1] + dr[2] * B.fyl[2]);
+        Gy[1] = A2 * (di[0] * B.fyl[0] + di[1] * B.fyl[1] + di[2] * B.fyl[2]);
     } else {
         // vertex half phases alpha_l / 2 in revolutions: (q . u_l) / (4 pi)
         float wr[3], wi[3];

@@ -9,7 +9,7 @@
 // Work decomposition: one lane per beam, 64 lanes per workgroup, all lanes of a
 // workgroup from one orientation and tree, so every lane reads the same bin at the same
 // time: the bin's position and Lam' come through scalar loads and stay in scalar
-// registers, the beam's constants and its 28 accumulators stay in vector registers, and
+// registers, the beam's constants and its 27 accumulators stay in vector registers, and
 // no reduction across lanes is needed.  A unit may cover a chunk of the bins only (when
 // there are too few beams to fill the chip); chunk partials are then summed in chunk
 // order.  Every sum has a fixed order: results repeat bit for bit.
@@ -63,8 +63,9 @@ __global__ __launch_bounds__(256) void k_adj_bins(DevGrid g, double* __restrict_
     o[5] = g.rad[bin]; o[6] = g.kr_rev[bin]; o[7] = 0.0;
 }
 
+// two waves per SIMD: the beam's constants and 27 FP64 accumulators live in registers
 template <int PREC>
-__global__ __launch_bounds__(AL) void k_adjoint(AdjArgs a)
+__global__ __launch_bounds__(AL, 2) void k_adjoint(AdjArgs a)
 {
     const AdjUnit U = a.units[blockIdx.x];
     const int lane = threadIdx.x;

@@ -115,6 +115,7 @@ __host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict_
     sincos_f64(bvk, &sb, &cb);
     double ar = 0.0, ai = 0.0;
     const auto edge = [&](double xj, double yj, double mj, double nj, double dx, double dy) {
+#pragma clang fp contract(off)
         const double delta = kxx * xj + kyy * yj;
         const double delta1 = kyy * mj + kxx;
         const double delta2 = kxx * nj + kyy;

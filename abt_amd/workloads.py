@@ -363,8 +363,14 @@ def column_beams(verts, faces, rot, la=0.532, m=1.31 + 0j, rec=8, seed=0, return
     t = trace_orientations(verts, faces, np.asarray(rot)[None], la=la, m=m, rec=rec)
     V = np.asarray(verts) @ np.asarray(rot).T
     b, e = to_beamset(t["beams"]), to_beamset(t["ext"])
+    bfi, efi = t["beams"]["fi"].numpy(), t["ext"]["fi"].numpy()
+    # apertures gathered from the rotated geometry itself, so that they are bit-identical
+    # to what a caller looking vertices up through the facet ids gets (diff_mod.f90:529-531)
+    F = np.asarray(faces)
+    b.verts = np.ascontiguousarray(V[F[bfi - 1]].reshape(-1, 3))
+    e.verts = np.ascontiguousarray(V[F[efi - 1]].reshape(-1, 3))
     if return_fi:
-        return b, e, V, t["beams"]["fi"].numpy(), t["ext"]["fi"].numpy()
+        return b, e, V, bfi, efi
     return b, e, V
 
 

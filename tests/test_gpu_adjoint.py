@@ -183,8 +183,9 @@ def test_vertex_gradient_matches_finite_differences():
     got, fd4, fd2 = map(np.array, (got, fd4, fd2))
     scale = np.abs(fd4).max()
     assert np.abs(got - fd4).max() <= 1e-6 * scale, (np.abs(got - fd4).max() / scale, got, fd4)
-    # the 2nd-order stencil carries the (q h)^2 truncation of the form-factor phases
-    assert np.abs(got - fd2).max() <= 1e-4 * scale
+    # the 2nd-order stencil carries the (q h)^2 / 6 truncation of the form-factor phases
+    # (|q| up to 2k = 23.6 /um): a few 1e-4 of the largest component
+    assert np.abs(got - fd2).max() <= 5e-4 * scale
 
 
 def test_amplitude_gradient_matches_finite_differences():
