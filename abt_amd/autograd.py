@@ -45,6 +45,20 @@ class FarFieldSystem:
         self.integrals = torch.empty(6 * self.norient, **f64)
         tri = lambda m: torch.arange(m + 1, dtype=torch.int64, device=self.device) * 3
         self.voff_b, self.voff_e = tri(self.nb), tri(self.ne)
+        # optional HIP-event timing of the library calls (forward kernel, adjoint kernel)
+        self.timing = False
+        self.events = {}
+
+    def _event_pair(self, name):
+        if not self.timing:
+            return None, None
+        e = (torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+        self.events[name] = e
+        return e
+
+    def kernel_ms(self, name):
+        e = self.events.get(name)
+        return e[0].elapsed_time(e[1]) if e else None
 
     def _prepare(self, verts, ampl, tree):
         d = self.beams if tree == 0 else self.ext
@@ -66,8 +80,13 @@ class FarFieldSystem:
         self._prepare(verts_b, ampl_b, 0)
         self._prepare(verts_e, ampl_e, 1)
         X = torch.empty((self.norient, 8, self.grid.nbins), dtype=torch.float64, device=self.device)
+        e0, e1 = self._event_pair("forward")
+        if e0 is not None:
+            e0.record()
         self.fwd.run(self.rec_b if self.nb else None, self.rec_e if self.ne else None, self.mueller,
                      self.integrals, ampl=X)
+        if e1 is not None:
+            e1.record()
         return X
 
 
@@ -89,9 +108,15 @@ class _FarField(torch.autograd.Function):
         gab = torch.empty((s.nb, 8), **f64)
         gve = torch.empty((s.ne, 3, 3), **f64)
         gae = torch.empty((s.ne, 8), **f64)
-        s.adj.run(rb if s.nb else None, re if s.ne else None, gX.contiguous(),
+        g = gX.contiguous()
+        e0, e1 = s._event_pair("adjoint")
+        if e0 is not None:
+            e0.record()
+        s.adj.run(rb if s.nb else None, re if s.ne else None, g,
                   gvb if s.nb else None, gab if s.nb else None, gve if s.ne else None,
                   gae if s.ne else None)
+        if e1 is not None:
+            e1.record()
         return None, gvb, gab, gve, gae
 
 

@@ -48,6 +48,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=96, help="apertures in the CPU baseline sample (0: skip)")
     ap.add_argument("--orient", type=int, default=1000,
                     help="orientations of the batched orientation-average workload (0: skip)")
+    ap.add_argument("--orient-fp64", type=int, default=1,
+                    help="run the FP64 plain and FP64 batched comparisons of the orientation average (0: skip)")
     ap.add_argument("--adjoint-orient", type=int, default=64,
                     help="orientations of the adjoint workload (0: skip)")
     ap.add_argument("--json-out", default=None)
@@ -163,7 +165,7 @@ def main():
     del recs, amp, amp_ext, mueller, work, raw
     torch.cuda.empty_cache()
     if args.orient > 0:
-        out["orientation_average"] = orientation_average(args.orient, args.steps, lam, dev)
+        out["orientation_average"] = orientation_average(args.orient, args.steps, lam, dev, bool(args.orient_fp64))
     if args.adjoint_orient > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_orient, lam, dev)
     line = json.dumps(out)
@@ -214,7 +216,7 @@ def _prepare_trees(w, lam, dev):
     return rb, re, nb, ne
 
 
-def orientation_average(norient, steps, lam, dev):
+def orientation_average(norient, steps, lam, dev, with_fp64=True):
     """Batched orientation averaging: `norient` orientations (uniform on SO(3), seeded) of
     a Gaussian-rough hexagonal column (radius 10 um, length 20 um, sigma 0.5 um, 2112
     facets), 10 internal reflections, 721 theta x 180 phi bins.  The whole job (every
@@ -268,6 +270,20 @@ def orientation_average(norient, steps, lam, dev):
     v_mixed = bb / (ms * 1e-3)
     log(f"  mixed batched (graph): {ms:.1f} ms/step, {v_mixed:.4e} beam.bin/s, tiles {tiles}")
     mm, Im = m.cpu().numpy().reshape(16, -1), I.cpu().numpy().reshape(-1, 6)
+    res = {
+        "workload": f"{norient} orientations (uniform SO(3), seed 1) of a rough hexagonal column "
+                    "(r 10 um, l 20 um, sigma 0.5 um, 2112 facets), m 1.31, 10 reflections, "
+                    "721 theta x 180 phi bins, lambda 0.532 um",
+        "beams": int(nb), "ext_beams": int(ne), "beams_per_orientation": [int(counts.min()), int(counts.max())],
+        "beam_bins": bb, "metric": "beam·bin diffractions/s", "value": v_mixed, "ms_per_step": ms,
+        "steps": steps, "launch": "one hipGraph replay per step (persistent kernel + reductions)",
+        "tiles": tiles, "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in tune.items()},
+        "roofline": {"bound": "valu", "achieved": F_ALG * bb / (ms * 1e-3) / 1e12, "peak": 157.3,
+                     "unit": "TFLOP/s", "frac": F_ALG * bb / (ms * 1e-3) / PEAK["mixed"]},
+    }
+    if not with_fp64:
+        plan.close()
+        return res
     # FP64 batched (accuracy reference)
     plan64 = lib.BatchPlan(g, boff, eoff, "fp64")
     m64, I64, m164 = buffers()
@@ -309,27 +325,18 @@ def orientation_average(norient, steps, lam, dev):
     s11r = np.abs(mm[0] - M64[0]) / s11
     q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
     asym_m, asym_64 = Im[:, 1] / Im[:, 0], I64h[:, 1] / I64h[:, 0]
-    res = {
-        "workload": f"{norient} orientations (uniform SO(3), seed 1) of a rough hexagonal column "
-                    "(r 10 um, l 20 um, sigma 0.5 um, 2112 facets), m 1.31, 10 reflections, "
-                    "721 theta x 180 phi bins, lambda 0.532 um",
-        "beams": int(nb), "ext_beams": int(ne), "beams_per_orientation": [int(counts.min()), int(counts.max())],
-        "beam_bins": bb, "metric": "beam·bin diffractions/s", "value": v_mixed, "ms_per_step": ms,
-        "steps": steps, "launch": "one hipGraph replay per step (persistent kernel + reductions)",
-        "tiles": tiles, "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in tune.items()},
+    res.update({
         "fp64_plain": {"value": v_plain, "ms": dt_plain * 1e3,
                        "path": "per orientation: abt_far_field_dev x2 + abt_finalise_dev + sum"},
         "fp64_batched": {"value": bb / (ms64b * 1e-3), "ms": ms64b},
         "speedup_vs_fp64_plain": v_mixed / v_plain,
-        "roofline": {"bound": "valu", "achieved": F_ALG * bb / (ms * 1e-3) / 1e12, "peak": 157.3,
-                     "unit": "TFLOP/s", "frac": F_ALG * bb / (ms * 1e-3) / PEAK["mixed"]},
         "accuracy": {"reference": "FP64 batched kernel on the same prepared beams",
                      "S11_rel": q(s11r), "norm_elem": q(norm),
                      "per_element_norm_max": [float(norm[e].max()) for e in range(16)],
                      "plain_vs_batched_fp64_norm_max": float((np.abs(Mp - M64) / s11).max()),
                      "asymmetry_abs_max": float(np.abs(asym_m - asym_64).max()),
                      "scatt_rel_max": float(np.abs(Im[:, 0] / I64h[:, 0] - 1).max())},
-    }
+    })
     for x in (plan, plan64):
         x.close()
     return res
@@ -371,6 +378,7 @@ def adjoint_bench(norient, lam, dev):
            "beams": nb, "ext_beams": ne, "beam_bins": (nb + ne) * g.nbins}
     for prec in ("mixed", "fp64"):
         s = mk(t, prec)
+        s.timing = True
         V = torch.tensor(verts, dtype=torch.float64, device=dev, requires_grad=True)
         ab = t["beams"]["ampl"].clone().requires_grad_(True)
         ae = t["ext"]["ampl"].clone().requires_grad_(True)
@@ -379,7 +387,7 @@ def adjoint_bench(norient, lam, dev):
             tr = t[tree]
             return torch.einsum("bij,bvj->bvi", R[tr["orient"]], V[facesT[tr["fi"] - 1]])
 
-        times = {"forward": [], "backward": []}
+        times = {"forward": [], "backward": [], "forward_kernel": [], "adjoint_kernel": []}
         for rep in range(3):
             V.grad = None
             ab.grad = ae.grad = None
@@ -398,10 +406,14 @@ def adjoint_bench(norient, lam, dev):
             if rep:
                 times["forward"].append(e[0].elapsed_time(e[1]))
                 times["backward"].append(e[2].elapsed_time(e[3]))
+                times["forward_kernel"].append(s.kernel_ms("forward"))
+                times["adjoint_kernel"].append(s.kernel_ms("adjoint"))
         grads = (V.grad, ab.grad, ae.grad)
         f, b = float(np.mean(times["forward"])), float(np.mean(times["backward"]))
+        fk, ak = float(np.mean(times["forward_kernel"])), float(np.mean(times["adjoint_kernel"]))
         out[prec] = {"forward_ms": f, "backward_ms": b, "backward_over_forward": b / f,
-                     "loss": float(L), "grad_vertex_norm": float(grads[0].norm())}
+                     "forward_kernel_ms": fk, "adjoint_kernel_ms": ak, "adjoint_over_forward_kernel": ak / fk,
+                     "loss": float(L.detach()), "grad_vertex_norm": float(grads[0].norm())}
         log(f"  adjoint {prec}: forward {f:.1f} ms, backward {b:.1f} ms, ratio {b / f:.2f}")
         if prec == "mixed":
             gm = grads[0].detach().cpu().numpy()
