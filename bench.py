@@ -132,12 +132,7 @@ def main():
     }
     ach = F_ALG * nb * nbins / (kms * 1e-3)
     kname = "k_far_field<%d,16>" % (1 if args.precision == "mixed" else 0)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath) and nb == 200_000:
-        # HBM bytes per launch of this kernel on this workload, from the committed
-        # rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel calibration
-        traffic = json.load(open(tpath)).get(kname, {}).get("bytes_per_launch")
+    traffic = _traffic(kname) if nb == 200_000 else None
     out["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK[args.precision] / 1e12,
                        "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": traffic,
                        "kernel": kname,
@@ -173,6 +168,15 @@ def main():
         with open(args.json_out, "w") as f:
             f.write(line + "\n")
     print(line, flush=True)
+
+
+def _traffic(kname):
+    """HBM bytes per launch of a kernel on its default workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes scaled by the copy-kernel calibration."""
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None
+    return json.load(open(tpath)).get(kname, {}).get("bytes_per_launch")
 
 
 def accuracy(am, a64, mm, m64, nbins, incoherent):
@@ -279,7 +283,10 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True):
         "steps": steps, "launch": "one hipGraph replay per step (persistent kernel + reductions)",
         "tiles": tiles, "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in tune.items()},
         "roofline": {"bound": "valu", "achieved": F_ALG * bb / (ms * 1e-3) / 1e12, "peak": 157.3,
-                     "unit": "TFLOP/s", "frac": F_ALG * bb / (ms * 1e-3) / PEAK["mixed"]},
+                     "unit": "TFLOP/s", "frac": F_ALG * bb / (ms * 1e-3) / PEAK["mixed"],
+                     "kernel": "k_batch<1,%d,%d>" % (tiles["beams_per_tile"], tiles["bins_per_thread"]),
+                     "traffic": _traffic("k_batch<1,%d,%d>" % (tiles["beams_per_tile"], tiles["bins_per_thread"]))
+                     if norient == 1000 else None},
     }
     if not with_fp64:
         plan.close()
