@@ -92,44 +92,55 @@ __global__ __launch_bounds__(MB) void k_phi_integrate(const double* M0, const do
     o[j + (int64_t)ntheta * e] = simpne_dev(nphi, phi, M + nb * e + (int64_t)nphi * j, 1);
 }
 
-// theta integrals (outputs_mod.f90:133-135, 198-205), one lane.
+// theta integrals (outputs_mod.f90:133-135, 198-205): the integrands of the three
+// matrices in parallel (one workgroup), then one lane per matrix runs simpne in the
+// reference's order.
 // params: scatt, scatt_beam, scatt_ext, g, g_beam, g_ext, back_scatt, f_beam, f_ext;
 // tmp: 3*ntheta scratch.  pass 0 with scaling: computes the energy scaling
 // factors f = sqrt(scatt_x / energy_x) into params[7..8].
-__global__ void k_theta_params(const double* __restrict__ m0, const double* __restrict__ m1,
-                               const double* __restrict__ m2, const double* __restrict__ M0,
-                               const double* __restrict__ theta, int ntheta, int nphi,
-                               double waveno, int scaling_pass, double ebeam, double eext,
-                               double* __restrict__ tmp, double* __restrict__ params)
+__global__ __launch_bounds__(MB) void k_theta_params(const double* __restrict__ m0, const double* __restrict__ m1,
+                                                     const double* __restrict__ m2, const double* __restrict__ M0,
+                                                     const double* __restrict__ theta, int ntheta, int nphi,
+                                                     double waveno, int scaling_pass, double ebeam, double eext,
+                                                     double* __restrict__ tmp, double* __restrict__ params)
 {
 #pragma clang fp contract(off)
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    __shared__ double sc[3];
+    const int tid = threadIdx.x;
     const double w2 = waveno * waveno;
     const double* m[3] = {m0, m1, m2};
-    double sc[3];
-    for (int k = 0; k < 3; k++) {
-        for (int j = 0; j < ntheta; j++) tmp[j] = m[k][j] * sin(theta[j]) / w2;
-        sc[k] = simpne_dev(ntheta, theta, tmp, 1);
+    for (int i = tid; i < 3 * ntheta; i += MB) {
+        const int k = i / ntheta, j = i - k * ntheta;
+        tmp[i] = m[k][j] * sin(theta[j]) / w2;
     }
+    __threadfence_block();
+    __syncthreads();
+    if (tid < 3) sc[tid] = simpne_dev(ntheta, theta, tmp + (int64_t)tid * ntheta, 1);
+    __syncthreads();
     if (scaling_pass) {
-        params[7] = sqrt(sc[1] / ebeam);
-        params[8] = sqrt(sc[2] / eext);
+        if (tid == 0) {
+            params[7] = sqrt(sc[1] / ebeam);
+            params[8] = sqrt(sc[2] / eext);
+        }
         return;
     }
-    double gg[3];
-    for (int k = 0; k < 3; k++) {
-        for (int j = 0; j < ntheta; j++) tmp[j] = m[k][j] * sin(theta[j]) * cos(theta[j]) / sc[k] / w2;
-        gg[k] = simpne_dev(ntheta, theta, tmp, 1);
+    for (int i = tid; i < 3 * ntheta; i += MB) {
+        const int k = i / ntheta, j = i - k * ntheta;
+        tmp[i] = m[k][j] * sin(theta[j]) * cos(theta[j]) / sc[k] / w2;
     }
-    int ib = 0;
-    double best = fabs(theta[0] - 2 * PI_R4);
-    for (int j = 1; j < ntheta; j++) {
-        const double d = fabs(theta[j] - 2 * PI_R4);
-        if (d < best) { best = d; ib = j; }
+    __threadfence_block();
+    __syncthreads();
+    if (tid < 3) params[3 + tid] = simpne_dev(ntheta, theta, tmp + (int64_t)tid * ntheta, 1);
+    if (tid == 0) {
+        int ib = 0;
+        double best = fabs(theta[0] - 2 * PI_R4);
+        for (int j = 1; j < ntheta; j++) {
+            const double d = fabs(theta[j] - 2 * PI_R4);
+            if (d < best) { best = d; ib = j; }
+        }
+        params[0] = sc[0]; params[1] = sc[1]; params[2] = sc[2];
+        params[6] = M0[(int64_t)nphi * ib] * 4 * PI_R4 / w2;
     }
-    params[0] = sc[0]; params[1] = sc[1]; params[2] = sc[2];
-    params[3] = gg[0]; params[4] = gg[1]; params[5] = gg[2];
-    params[6] = M0[(int64_t)nphi * ib] * 4 * PI_R4 / w2;
 }
 
 __global__ __launch_bounds__(MB) void k_accumulate(double* __restrict__ total,
@@ -169,11 +180,11 @@ extern "C" int abt_launch_finalise(const abt::DevGrid* g, double* ab,
                            (const double*)M0, (const double*)M1, (const double*)M2, g->phi, nph, nth,
                            o0, o1, o2);
         if (scaling && pass == 0)
-            hipLaunchKernelGGL(k_theta_params, dim3(1), dim3(64), 0, st, (const double*)o0,
+            hipLaunchKernelGGL(k_theta_params, dim3(1), dim3(MB), 0, st, (const double*)o0,
                                (const double*)o1, (const double*)o2, (const double*)M0, g->theta,
                                nth, nph, g->waveno, 1, ebeam, eext, tmp, params);
     }
-    hipLaunchKernelGGL(k_theta_params, dim3(1), dim3(64), 0, st, (const double*)o0, (const double*)o1,
+    hipLaunchKernelGGL(k_theta_params, dim3(1), dim3(MB), 0, st, (const double*)o0, (const double*)o1,
                        (const double*)o2, (const double*)M0, g->theta, nth, nph, g->waveno, 0,
                        ebeam, eext, tmp, params);
     return (int)hipGetLastError();
