@@ -2,7 +2,11 @@
 // aperture part of contour_integral, diff_mod.f90:334-354), shared by the host
 // preparation (glibc libm, bit-reproducible against a CPU FP64 run) and the
 // device preparation kernel.  Operation order follows the reference; FMA
-// contraction is off so host and device round each operation alike.
+// contraction is off so host and device round each operation alike.  Cosine and
+// sine of one angle come from one sincos call: that is what gfortran -O3 makes of
+// the reference's pairs, and glibc's sincos differs from separate cos and sin in
+// the last bit for a few arguments in a thousand -- enough to change the frame of
+// an aperture at normal incidence, whose in-plane angle is atan2 of rounding noise.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <float.h>
@@ -36,18 +40,18 @@ __host__ __device__ inline void m3_vec(const M3& x, const double v[3], double ou
 __host__ __device__ inline void aperture_rotation(const double a1[3], const double b1[3], M3& rot)
 {
 #pragma clang fp contract(off)
-    double th1 = atan(a1[0] / a1[1]);
-    double c1 = cos(th1), s1 = sin(th1);
+    double th1 = atan(a1[0] / a1[1]), c1, s1;
+    sincos(th1, &s1, &c1);
     M3 r1 = {{{c1, -s1, 0}, {s1, c1, 0}, {0, 0, 1}}};
     double a2[3], b2[3];
     m3_vec(r1, a1, a2); m3_vec(r1, b1, b2);
-    double th2 = -atan(a2[2] / a2[1]);
-    double c2 = cos(th2), s2 = sin(th2);
+    double th2 = -atan(a2[2] / a2[1]), c2, s2;
+    sincos(th2, &s2, &c2);
     M3 r2 = {{{1, 0, 0}, {0, c2, -s2}, {0, s2, c2}}};
     double a3[3], b3[3];
     m3_vec(r2, a2, a3); m3_vec(r2, b2, b3);
-    double th3 = atan(b3[2] / b3[0]);
-    double c3 = cos(th3), s3 = sin(th3);
+    double th3 = atan(b3[2] / b3[0]), c3, s3;
+    sincos(th3, &s3, &c3);
     M3 r3 = {{{c3, 0, s3}, {0, 1, 0}, {-s3, 0, c3}}};
     double a4[3], b4[3];
     m3_vec(r3, a3, a4); m3_vec(r3, b3, b4);
@@ -90,7 +94,8 @@ __host__ __device__ inline void make_beam_record(const double* ampl, const doubl
     m3_vec(rot, prop0, prop1);
     m3_vec(rot, perp0, perp1);
     double angle = -atan2(prop1[1], prop1[0]);
-    double ca = cos(angle), sa = sin(angle);
+    double ca, sa;
+    sincos(angle, &sa, &ca);
     M3 rot2 = {{{ca, -sa, 0}, {sa, ca, 0}, {0, 0, 1}}};
     double prop2[3], perp2[3];
     m3_vec(rot2, prop1, prop2);

@@ -185,14 +185,18 @@ int abt_ensure_device(int* cus)
 static void host_bins(int64_t nth, const double* th, int64_t nph, const double* ph, double* x,
                       double* y, double* z)
 {
-    // make_far_field_bins (diff_mod.f90:827-855), meshgrid layout (nphi, ntheta)
+    // make_far_field_bins (diff_mod.f90:827-855), meshgrid layout (nphi, ntheta); the
+    // sine and cosine of one angle from one sincos call, as gfortran -O3 computes them
     const double r = 1e6;
     for (int64_t j = 0; j < nth; j++)
         for (int64_t i = 0; i < nph; i++) {
             const int64_t b = i + nph * j;
-            x[b] = r * sin(th[j]) * cos(ph[i]);
-            y[b] = r * sin(th[j]) * sin(ph[i]);
-            z[b] = -(r * cos(th[j]));
+            double st, ct, sp, cp;
+            sincos(th[j], &st, &ct);
+            sincos(ph[i], &sp, &cp);
+            x[b] = r * st * cp;
+            y[b] = r * st * sp;
+            z[b] = -(r * ct);
         }
 }
 
@@ -225,10 +229,7 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
         rad[b] = r;
         krr[b] = krev * r;
     }
-    for (int64_t i = 0; i < nph; i++) {
-        cp[i] = cos(ph[i]);
-        sp[i] = sin(ph[i]);
-    }
+    for (int64_t i = 0; i < nph; i++) sincos(ph[i], &sp[i], &cp[i]);
     abt_dev_grid* g = new abt_dev_grid();
     if (hipMalloc(&g->mem, sizeof(double) * h.size()) != hipSuccess) {
         delete g;

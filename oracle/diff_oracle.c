@@ -22,7 +22,12 @@
  *   finalise              outputs_mod.f90:63-278 (numerical part)
  *
  * Build with -ffp-contract=off (no FMA contraction, like the reference's
- * baseline x86-64 build) and glibc libm.
+ * baseline x86-64 build) and glibc libm.  Where the reference takes the cosine
+ * and the sine of one argument, gfortran -O3 (the reference's Makefiles) turns
+ * the pair into one glibc sincos call, which differs from separate sin and cos
+ * calls in the last bit for a few arguments in a thousand; this file calls
+ * sincos explicitly at those places so that its results do not depend on the
+ * optimisation level it is built with.
  *
  * Array conventions (all match the Fortran column-major layouts):
  *   grid arrays x,y,z     (nphi, ntheta)        index i + nphi*j
@@ -44,6 +49,7 @@
  * (diff_mod.f90:630-635) have no independent pin: parity unpinned for those
  * two conventions beyond code reading.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -104,9 +110,12 @@ void oracle_make_far_field_bins(int64_t nphi, int64_t ntheta, const double *thet
     for (int64_t j = 0; j < ntheta; j++)
         for (int64_t i = 0; i < nphi; i++) {
             int64_t b = i + nphi * j;
-            x[b] = r * sin(theta[j]) * cos(phi[i]);
-            y[b] = r * sin(theta[j]) * sin(phi[i]);
-            z[b] = -(r * cos(theta[j]));
+            double st, ct, sp, cp;
+            sincos(theta[j], &st, &ct);
+            sincos(phi[i], &sp, &cp);
+            x[b] = r * st * cp;
+            y[b] = r * st * sp;
+            z[b] = -(r * ct);
         }
 }
 
@@ -116,16 +125,19 @@ void oracle_get_rotation_matrix2(const double *v0, double rot_out[9])
 {
     double a1[3] = {v0[0], v0[1], v0[2]};
     double b1[3] = {v0[3], v0[4], v0[5]};
-    double th1 = atan(a1[0] / a1[1]);
-    double r1[3][3] = {{cos(th1), -sin(th1), 0}, {sin(th1), cos(th1), 0}, {0, 0, 1}};
+    double th1 = atan(a1[0] / a1[1]), s1, c1;
+    sincos(th1, &s1, &c1);
+    double r1[3][3] = {{c1, -s1, 0}, {s1, c1, 0}, {0, 0, 1}};
     double a2[3], b2[3];
     mat3_vec(r1, a1, a2); mat3_vec(r1, b1, b2);
-    double th2 = -atan(a2[2] / a2[1]);
-    double r2[3][3] = {{1, 0, 0}, {0, cos(th2), -sin(th2)}, {0, sin(th2), cos(th2)}};
+    double th2 = -atan(a2[2] / a2[1]), s2, c2;
+    sincos(th2, &s2, &c2);
+    double r2[3][3] = {{1, 0, 0}, {0, c2, -s2}, {0, s2, c2}};
     double a3[3], b3[3];
     mat3_vec(r2, a2, a3); mat3_vec(r2, b2, b3);
-    double th3 = atan(b3[2] / b3[0]);
-    double r3[3][3] = {{cos(th3), 0, sin(th3)}, {0, 1, 0}, {-sin(th3), 0, cos(th3)}};
+    double th3 = atan(b3[2] / b3[0]), s3, c3;
+    sincos(th3, &s3, &c3);
+    double r3[3][3] = {{c3, 0, s3}, {0, 1, 0}, {-s3, 0, c3}};
     double a4[3], b4[3];
     mat3_vec(r3, a3, a4); mat3_vec(r3, b3, b4);
     double t21[3][3], t321[3][3], rot[3][3];
@@ -206,8 +218,9 @@ static void beam_frame_setup(beam_frame *bf, const double *ampl, const double *v
     double prop1[3], perp1[3];
     mat3_vec(bf->rot, prop0, prop1);
     mat3_vec(bf->rot, perp0, perp1);
-    double angle = -atan2(prop1[1], prop1[0]);
-    double r2[3][3] = {{cos(angle), -sin(angle), 0}, {sin(angle), cos(angle), 0}, {0, 0, 1}};
+    double angle = -atan2(prop1[1], prop1[0]), sa, ca;
+    sincos(angle, &sa, &ca);
+    double r2[3][3] = {{ca, -sa, 0}, {sa, ca, 0}, {0, 0, 1}};
     memcpy(bf->rot2, r2, sizeof(r2));
     mat3_vec(bf->rot2, prop1, bf->prop2);
     mat3_vec(bf->rot2, perp1, bf->perp2);
@@ -279,7 +292,8 @@ static void contour_bin(const contour_ctx *c, double x3, double y3, double z3,
     double kyy = c->kinc[1] - w * w * y3 / bvk;
     if (fabs(kxx) < KMIN) kxx = KMIN;
     if (fabs(kyy) < KMIN) kyy = KMIN;
-    double cb = cos(bvk), sb = sin(bvk);
+    double cb, sb;
+    sincos(bvk, &sb, &cb);
     double ar = 0.0, ai = 0.0;
     int64_t nv = c->nv;
     for (int64_t j = 0; j < nv; j++) {
@@ -298,8 +312,12 @@ static void contour_bin(const contour_ctx *c, double x3, double y3, double z3,
         double omega2 = dy * delta2;
         double alpha = 1 / (2 * kyy * delta1);
         double beta = 1 / (2 * kxx * delta2);
-        double sumim = alpha * (cos(delta) - cos(delta + omega1)) - beta * (cos(delta) - cos(delta + omega2));
-        double sumre = -(alpha * (sin(delta) - sin(delta + omega1))) + beta * (sin(delta) - sin(delta + omega2));
+        double s0, c0, s1, c1, s2, c2;
+        sincos(delta, &s0, &c0);
+        sincos(delta + omega1, &s1, &c1);
+        sincos(delta + omega2, &s2, &c2);
+        double sumim = alpha * (c0 - c1) - beta * (c0 - c2);
+        double sumre = -(alpha * (s0 - s1)) + beta * (s0 - s2);
         double pr = cb * sumre - sb * sumim;
         double pi = cb * sumim + sb * sumre;
         ar = ar + pr / c->lambda;
@@ -430,7 +448,7 @@ static void phi_tables(int64_t nphi, const double *phi, double **cphi, double **
 {
     *cphi = (double *)malloc(sizeof(double) * 2 * (size_t)nphi);
     *sphi = *cphi + nphi;
-    for (int64_t i = 0; i < nphi; i++) { (*cphi)[i] = cos(phi[i]); (*sphi)[i] = sin(phi[i]); }
+    for (int64_t i = 0; i < nphi; i++) sincos(phi[i], &(*sphi)[i], &(*cphi)[i]);
 }
 
 /* One call of `diffraction` for a single beam. amplC fully overwritten. */
