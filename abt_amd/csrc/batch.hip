@@ -54,19 +54,39 @@ __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restri
         const double* src = recs + t0 * REC;
         for (int q = tid; q < nt * REC; q += BB) tile[q] = src[q];
         __syncthreads();
-        for (int bb = 0; bb < nt; bb++) {
+        if (PREC == 1) {
+            // FP32 partial sums over groups of MIXED_GROUP beams (counted from the
+            // orientation's first beam; TB is a multiple), added into FP64
+            for (int g0 = 0; g0 < nt; g0 += MIXED_GROUP) {
+                float accf[BPT][8];
 #pragma unroll
-            for (int j = 0; j < BPT; j++) {
-                if (PREC == 0)
-                    beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
-                else if (PREC == 3) {
-                    AdjBeam B;
-                    adj_beam_from_record(tile + bb * REC, B);
-                    model_bin_fp64(B, bi[j].x, bi[j].y, bi[j].z, bi[j].xt, bi[j].yt, bi[j].rad, a.g.waveno,
-                                   a.g.lambda, acc[j]);
+                for (int j = 0; j < BPT; j++)
+#pragma unroll
+                    for (int q = 0; q < 8; q++) accf[j][q] = 0.f;
+                const int g1 = min(nt, g0 + MIXED_GROUP);
+                for (int bb = g0; bb < g1; bb++) {
+#pragma unroll
+                    for (int j = 0; j < BPT; j++)
+                        beam_bin_mixed(tile + bb * REC, vrecs, a.g, bi[j], a.fast_diff, accf[j]);
                 }
-                else
-                    beam_bin_mixed(tile + bb * REC, vrecs, a.g, bi[j], a.fast_diff, acc[j]);
+#pragma unroll
+                for (int j = 0; j < BPT; j++)
+#pragma unroll
+                    for (int q = 0; q < 8; q++) acc[j][q] += (double)accf[j][q];
+            }
+        } else {
+            for (int bb = 0; bb < nt; bb++) {
+#pragma unroll
+                for (int j = 0; j < BPT; j++) {
+                    if (PREC == 0)
+                        beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
+                    else {
+                        AdjBeam B;
+                        adj_beam_from_record(tile + bb * REC, B);
+                        model_bin_fp64(B, bi[j].x, bi[j].y, bi[j].z, bi[j].xt, bi[j].yt, bi[j].rad, a.g.waveno,
+                                       a.g.lambda, acc[j]);
+                    }
+                }
             }
         }
     }

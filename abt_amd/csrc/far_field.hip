@@ -71,13 +71,20 @@ __global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __
         for (int q = tid; q < nt * REC; q += BLOCK) tile[q] = src[q];
         __syncthreads();
         if (active) {
-            for (int bb = 0; bb < nt; bb++) {
-                if (PREC == 0)
-                    beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc);
-                else if (PREC == 1)
-                    beam_bin_mixed(tile + bb * REC, vrecs, g, bi, fast_diff, acc);
-                else
-                    beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, true);
+            if (PREC == 1) {
+                // FP32 partial sums over groups of MIXED_GROUP beams, added into FP64
+                for (int g0 = 0; g0 < nt; g0 += MIXED_GROUP) {
+                    float accf[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) accf[q] = 0.f;
+                    const int g1 = min(nt, g0 + MIXED_GROUP);
+                    for (int bb = g0; bb < g1; bb++) beam_bin_mixed(tile + bb * REC, vrecs, g, bi, fast_diff, accf);
+#pragma unroll
+                    for (int q = 0; q < 8; q++) acc[q] += (double)accf[q];
+                }
+            } else {
+                for (int bb = 0; bb < nt; bb++)
+                    beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, PREC == 2);
             }
         }
     }
@@ -141,7 +148,12 @@ extern "C" int abt_launch_far_field(const abt::DevGrid* g, const double* recs, c
 {
     using namespace abt;
     if (g->nbins <= 0) return 0;
-    const int64_t per_chunk = (nb + nchunks - 1) / nchunks;
+    // chunk lengths are multiples of the largest tile (32 beams), so the mixed kernel's
+    // FP32 beam groups do not depend on the chunking
+    int64_t per_chunk = (nb + nchunks - 1) / nchunks;
+    per_chunk = per_chunk < 32 ? 32 : (per_chunk + 31) / 32 * 32;
+    nchunks = (int)((nb + per_chunk - 1) / per_chunk);
+    if (nchunks < 1) nchunks = 1;
     dim3 grid((g->nbins + BLOCK - 1) / BLOCK, nchunks);
     double* w = nchunks > 1 ? work : nullptr;
 #define L(P, T) hipLaunchKernelGGL((k_far_field<P, T>), grid, dim3(BLOCK), 0, st, *g, recs, vrecs, nb, per_chunk, fast_diff, out, w)

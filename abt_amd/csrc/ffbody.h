@@ -217,23 +217,48 @@ __host__ __device__ __forceinline__ float cos_poly(float z)
                 ABT_KR0);
 }
 
-// sinc of an edge, sin(2 pi h) / (2 pi h) for the half phase difference h
-// (revolutions, FP64), up to the sign (-1)^m returned in *sgn.  For m = 0
-// (|h| <= 1/4) the quotient is P(r^2) / (2 pi) with no cancellation.
-__host__ __device__ __forceinline__ float sinc_half(double h, unsigned* sgn)
+// Polynomials in half revolutions: for |x| <= 1/2 and z = x^2,
+//   sinpi_over(z) = sin(pi x) / x,  cospi_poly(z) = cos(pi x),  sincpi_poly(z) = sin(pi x) / (pi x);
+// the first two are sin_over_r / cos_poly with the argument doubled (coefficients scaled
+// by powers of two, so the values are the same to the last bit).
+__host__ __device__ __forceinline__ float sinpi_over(float z)
 {
-    const double m = rint(2.0 * h);
-    *sgn = ((unsigned)(int)m) << 31;
-    const float r = (float)fma(-0.5, m, h);
-    const float ps = sin_over_r(r * r);
-    const float small = ps * 0.15915494309189535f;
-    const float big = (r * ps) * abt_rcpf(6.283185307179586f * (float)h);
-    return (m == 0.0) ? small : big;
+    return fmaf(z, fmaf(z, fmaf(z, fmaf(z, ABT_HR4 / 512.f, ABT_HR3 / 128.f), ABT_HR2 / 32.f), ABT_HR1 / 8.f),
+                ABT_HR0 / 2.f);
+}
+__host__ __device__ __forceinline__ float cospi_poly(float z)
+{
+    return fmaf(z, fmaf(z, fmaf(z, fmaf(z, fmaf(z, ABT_KR5 / 1024.f, ABT_KR4 / 256.f), ABT_KR3 / 64.f),
+                                ABT_KR2 / 16.f), ABT_KR1 / 4.f), ABT_KR0);
+}
+__host__ __device__ __forceinline__ float sincpi_poly(float z)
+{
+    constexpr double c = 0.15915494309189535;   // 1 / 2 pi
+    return fmaf(z, fmaf(z, fmaf(z, fmaf(z, (float)(ABT_HR4 * c / 256.), (float)(ABT_HR3 * c / 64.)),
+                                (float)(ABT_HR2 * c / 16.)), (float)(ABT_HR1 * c / 4.)), (float)(ABT_HR0 * c));
 }
 
-constexpr double SERIES_QA = 1.5;    // |q_perp| * a (rad) below which the FP64 series is used
-constexpr float SERIES_QA2 = 2.25f;  // SERIES_QA^2
-constexpr int SERIES_N = 24;         // truncation error < 1e-19 relative at |q| a = 1.5
+// t (FP64, half revolutions: e^{i pi t}) reduced to |x| <= 1/2: e^{i pi t} = (-1)^m e^{i pi x},
+// m = rint(t); the parity of m comes back as an FP32 sign mask (bit 31).
+__host__ __device__ __forceinline__ float pi_reduce(double t, unsigned* sgn)
+{
+    const double m = rint(t);
+    *sgn = ((unsigned)(int)m) << 31;
+    return (float)(t - m);
+}
+
+// e^{i pi t} in FP32 from the FP64 t
+__host__ __device__ __forceinline__ void expi_pi(double t, float* c, float* s)
+{
+    unsigned sg;
+    const float x = pi_reduce(t, &sg);
+    const float z = x * x;
+    *s = flipsign(x * sinpi_over(z), sg);
+    *c = flipsign(cospi_poly(z), sg);
+}
+
+constexpr float SERIES_QA2 = 2.25f;  // (|q_perp| a)^2 below which the FP64 series is used
+constexpr int SERIES_N = 16;         // truncation < 3e-12 relative at |q| a = 1.5
 
 // 2 * sum_n i^n h_n(a,b,c) / (n+2)!  (triangle form factor / area, vertex phases a,b,c rad)
 __host__ __device__ __forceinline__ void tri_series(double a, double b, double c, double* re, double* im)
@@ -266,30 +291,31 @@ __host__ __device__ __forceinline__ void tri_series(double a, double b, double c
     *im = 2.0 * si;
 }
 
-
-// Edge term of the rotation-invariant form factor: c_j sinc(K_j/2) e^{i mu_j},
-// with half vertex phases ha, hb (revolutions, FP64).
-__host__ __device__ __forceinline__ void edge_term(double ha, double hb, float fqx, float fqy,
-                                                   float edx, float edy, float& ssr, float& ssi)
+// One edge of the rotation-invariant form factor with the far-field phase folded in:
+// c_j sinc(pi d) e^{i (mu_j + k|r - com|)}, from the vertex exponentials w = e^{i pi t'}
+// (t' = q.v + k|r - com|/2pi, half revolutions) of its two ends; d = t'_{j+1} - t'_j.
+__host__ __device__ __forceinline__ void edge_exp(float wr0, float wi0, float wr1, float wi1, float d, float fqx,
+                                                  float fqy, float edx, float edy, float& ssr, float& ssi)
 {
-    unsigned s1, s2;
-    const float r = half_reduce(ha + hb, &s1);     // mu_j = (delta_j + delta_{j+1}) / 2
-    const float sc = sinc_half(hb - ha, &s2);      // (delta_{j+1} - delta_j) / 2
-    const float z = r * r;
-    const float sm = r * sin_over_r(z), cm = cos_poly(z);
-    const float t = flipsign(fmaf(fqx, edy, -fqy * edx) * sc, s1 ^ s2);
-    ssr = fmaf(t, cm, ssr);
-    ssi = fmaf(t, sm, ssi);
+    const float Er = fmaf(wr0, wr1, -wi0 * wi1);
+    const float Ei = fmaf(wr0, wi1, wi0 * wr1);
+    const float sx = fmaf(wi1, wr0, -wr1 * wi0);        // sin(pi d) = Im(w1 conj(w0))
+    const float S = fabsf(d) < 0.5f ? sincpi_poly(d * d) : sx * abt_rcpf(3.14159265358979f * d);
+    const float t = fmaf(fqx, edy, -fqy * edx) * S;
+    ssr = fmaf(t, Er, ssr);
+    ssi = fmaf(t, Ei, ssi);
 }
 
-// Mixed-precision body: one beam, one bin.  acc holds S+ (acc[0..3]) and S-
-// (acc[4..7]), the sums of w r+ and w' r- over beams (see mixed_finish).
+// Mixed-precision body: one beam, one bin.  Adds the beam's contribution to the FP32
+// projector sums accf: S+ (accf[0..3]) and S- (accf[4..7]), see mixed_finish; callers
+// add accf into FP64 sums every few beams.
 __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict__ r,
                                                         const double* __restrict__ vrecs,
                                                         const DevGrid& g, const BinIn& bi,
-                                                        int fast_diff, double acc[8])
+                                                        int fast_diff, float accf[8])
 {
     const float* rf = reinterpret_cast<const float*>(r + R_F32);
+    const float* rg = reinterpret_cast<const float*>(r + R_G32);
     // ---- FP64 geometry: bin direction from the aperture centre, |r - com| ----
     const double dx = bi.x - r[R_COM], dy = bi.y - r[R_COM + 1], dz = bi.z - r[R_COM + 2];
     const double s = fma(dx, dx, fma(dy, dy, dz * dz));
@@ -302,22 +328,18 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     if (fast_diff) {
         if (ux * r[R_PROP0] + uy * r[R_PROP0 + 1] + uz * r[R_PROP0 + 2] < r[R_COSFOV]) return;
     }
-    const double* R = r + R_RR;
-    const double k1 = fma(R[0], ux, fma(R[1], uy, R[2] * uz));
-    const double k2 = fma(R[3], ux, fma(R[4], uy, R[5] * uz));
-    const double k3 = fma(R[6], ux, fma(R[7], uy, R[8] * uz));
-    // q = k (prop2 - khat) in the aperture plane, revolutions per um
-    const double qx = g.krev * (r[R_PROP2] - k1);
-    const double qy = g.krev * (r[R_PROP2 + 1] - k2);
-    unsigned sb_;
-    const float rb = half_reduce(g.krev * (s * invd), &sb_);   // e^{i k |r - com|}
-    const float fsb = flipsign(rb * sin_over_r(rb * rb), sb_);
-    const float fcb = flipsign(cos_poly(rb * rb), sb_);
+    // q = krev (prop2 - khat) in the aperture plane (revolutions per um); krev khat_x,y
+    const double* K = r + R_KRR;
+    const double qx = fma(-K[0], ux, fma(-K[1], uy, fma(-K[2], uz, r[R_KP2])));
+    const double qy = fma(-K[3], ux, fma(-K[4], uy, fma(-K[5], uz, r[R_KP2 + 1])));
+    const double k1r = r[R_KP2] - qx, k2r = r[R_KP2 + 1] - qy;
+    const double kd = g.krev * (s * invd);               // k |r - com| / 2 pi
     const float fqx = (float)qx, fqy = (float)qy;
     const float q2 = fmaf(fqx, fqx, fqy * fqy);
+    const float il = g.inv_lambda_f;
 
-    // ---- aperture integral: sum_ref (the reference's edge sum) ----
-    float fr_, fi_;
+    // ---- aperture integral with the far-field phase: (fR, fI) = e^{i k|r-com|} sum_ref / lambda ----
+    float fR, fI;
     const bool tri = rf[F_NV] == 3.0f;
     if (q2 * rf[F_QA2] < SERIES_QA2) {
         // FP64 series of the form factor about q = 0; sum_ref = -i * (signed-area F)
@@ -347,90 +369,131 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
                 si += sa * ti;
             }
         }
-        fr_ = (float)si;
-        fi_ = (float)(-sr);
+        const float fr_ = (float)si, fi_ = (float)(-sr);
+        float cb, sb;
+        expi_pi(2.0 * kd, &cb, &sb);
+        fR = (cb * fr_ - sb * fi_) * il;
+        fI = (cb * fi_ + sb * fr_) * il;
     } else {
+        // sum_ref = -(1/2pi) sum_j c_j sinc(pi d_j) e^{i mu_j} / |q|^2 (Fraunhofer, rotation-invariant form)
         float ssr = 0.f, ssi = 0.f;
         if (tri) {
-            const double h0 = fma(qx, r[R_HX], qy * r[R_HY]);
-            const double h1 = fma(qx, r[R_HX + 1], qy * r[R_HY + 1]);
-            const double h2 = fma(qx, r[R_HX + 2], qy * r[R_HY + 2]);
-            edge_term(h0, h1, fqx, fqy, rf[F_EDX], rf[F_EDY], ssr, ssi);
-            edge_term(h1, h2, fqx, fqy, rf[F_EDX + 1], rf[F_EDY + 1], ssr, ssi);
-            edge_term(h2, h0, fqx, fqy, rf[F_EDX + 2], rf[F_EDY + 2], ssr, ssi);
+            double t[3];
+            float wr[3], wi[3];
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                t[l] = fma(qx, r[R_EX + l], fma(qy, r[R_EY + l], kd));
+                expi_pi(t[l], &wr[l], &wi[l]);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int jn = j == 2 ? 0 : j + 1;
+                edge_exp(wr[j], wi[j], wr[jn], wi[jn], (float)(t[jn] - t[j]), fqx, fqy, rf[F_EDX + j],
+                         rf[F_EDY + j], ssr, ssi);
+            }
         } else {
             const int nv = (int)rf[F_NV];
             const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
-            const double hfirst = 0.5 * fma(qx, vr[V_X], qy * vr[V_Y]);
-            double hprev = hfirst;
+            const double t0 = fma(qx, vr[V_X], fma(qy, vr[V_Y], kd));
+            float wr0, wi0;
+            expi_pi(t0, &wr0, &wi0);
+            double tp = t0;
+            float wrp = wr0, wip = wi0;
             for (int j = 0; j < nv; j++) {
-                const double hn = (j == nv - 1) ? hfirst
-                                                : 0.5 * fma(qx, vr[VREC * (j + 1) + V_X], qy * vr[VREC * (j + 1) + V_Y]);
-                edge_term(hprev, hn, fqx, fqy, (float)vr[VREC * j + V_DX], (float)vr[VREC * j + V_DY], ssr, ssi);
-                hprev = hn;
+                double tn = t0;
+                float wrn = wr0, win = wi0;
+                if (j < nv - 1) {
+                    tn = fma(qx, vr[VREC * (j + 1) + V_X], fma(qy, vr[VREC * (j + 1) + V_Y], kd));
+                    expi_pi(tn, &wrn, &win);
+                }
+                edge_exp(wrp, wip, wrn, win, (float)(tn - tp), fqx, fqy, (float)vr[VREC * j + V_DX],
+                         (float)vr[VREC * j + V_DY], ssr, ssi);
+                tp = tn;
+                wrp = wrn;
+                wip = win;
             }
         }
-        // sum_ref = -(1/2pi) sum / |q|^2
-        const float sc = -0.15915494309189535f * abt_rcpf(q2);
-        fr_ = ssr * sc;
-        fi_ = ssi * sc;
+        const float sc = (-0.15915494309189535f * il) * abt_rcpf(q2);
+        fR = ssr * sc;
+        fI = ssi * sc;
     }
-    // area_facs2 = e^{i bvk} * sum_ref / lambda
-    const float il = g.inv_lambda_f;
-    const float fR = (fcb * fr_ - fsb * fi_) * il;
-    const float fI = (fcb * fi_ + fsb * fr_) * il;
 
     // ---- polarisation, FP32 (karczewski + rot4 of diff_mod.f90:124-198, 630-643) ----
-    // the |k_y| > 0.999999 clamp (diff_mod.f90:151) is decided in FP64, as the reference does
-    const bool kclamp = fabs(k2) > K2CLAMP;
-    const float fk1 = (float)k1, fk3 = (float)k3;
-    const float fk2 = kclamp ? copysignf(0.99999898672103882f, (float)k2) : (float)k2;
-    const float s2 = fmaf(fk1, fk1, fk3 * fk3);                // 1 - k2^2 (unclamped)
-    const float s2c = kclamp ? 2.02655679e-06f : s2;           // clamped: 1 - 0.999999^2
-    const float rs = abt_rsqf(s2c);
-    const float sq = s2c * rs;                                   // sqrt(1 - k2^2)
-    const float frac = sq * rf[F_SIG];
-    const float ifrac = rs * rf[F_ISIG];
-    const float da = -0.5f * fmaf(rf[F_KZ], frac, fk3 * ifrac);          // diff_ampl(1,1) = (2,2)
-    const float db = 0.5f * fmaf(rf[F_KXY], frac, -fk1 * fk2 * ifrac);   // diff_ampl(1,2) = -(2,1)
-    // m and evo2 = unit(k x m) in closed form
-    const float m1 = -fk1 * fk2 * rs, m2 = sq, m3 = -fk2 * fk3 * rs;
-    const float r13 = kclamp ? abt_rsqf(s2) : rs;
-    const float ev1 = -fk3 * r13, ev3 = fk1 * r13;
-    // hc = rot2.rot . unit(inc x khat), inc = (0,0,-1): global (u_y, -u_x, 0);
-    // forward fallback (diff_mod.f90:630-635): the bin (same phi, theta_i)
-    float hx = (float)uy, hy = (float)(-ux);
-    if (!(fabs(uz) < FWD_DOT)) {
-        hx = (float)(bi.yt - r[R_COM + 1]);
-        hy = (float)(-(bi.xt - r[R_COM]));
+    // rot4 . diff_ampl = gg I + hh J, J = [[0,1],[-1,0]]; the |k_y| > 0.999999 clamp
+    // (diff_mod.f90:151) is decided in FP64, as the reference does
+    const bool kclamp = fabs(k2r) > g.krev_k2clamp;
+    const bool fwd = !(fabs(uz) < FWD_DOT);
+    const float ik = g.inv_krev_f;
+    const float fk1 = (float)k1r * ik, fk2 = (float)k2r * ik;
+    const float fk3 = fmaf(rf[F_RR + 6], (float)ux, fmaf(rf[F_RR + 7], (float)uy, rf[F_RR + 8] * (float)uz));
+    const float s2 = fmaf(fk1, fk1, fk3 * fk3);                   // 1 - k_y^2 (unclamped)
+    float gg, hh;
+    if (!kclamp && !fwd) {
+        // hc = unit(inc x khat) is normal to khat, so with m and k x m of karczewski:
+        // hc.m = h_y / sqrt(1 - k_y^2), hc.unit(k x m) = (hc x khat)_y / sqrt(1 - k_y^2), where
+        // |inc x khat| h_y = krow2 . (u_y, -u_x, 0) and |inc x khat| (hc x khat)_y = rr_23 - u_z k_y
+        // (aperture frame, inc = (0,0,-1)); FP64 here, both vanish at k = +-e_y
+        const double P0 = fma(K[3], uy, -K[4] * ux);       // krev |inc x khat| h_y
+        const double Q0 = fma(-uz, k2r, K[5]);             // krev |inc x khat| (hc x khat)_y
+        const float fux = (float)ux, fuy = (float)uy;
+        const float rh = abt_rsqf(fmaf(fux, fux, fuy * fuy)) * abt_rcpf(s2);
+        const float P = (float)P0 * rh, Q = (float)Q0 * rh;
+        // diff_ampl(1,1) = (2,2) = A / sqrt(1 - k_y^2), diff_ampl(1,2) = -(2,1) = B / sqrt(1 - k_y^2)
+        const float A = fmaf(rg[G_C1], s2, rg[G_C2] * fk3);
+        const float B = fmaf(rg[G_C3], s2, rg[G_C2] * (fk1 * fk2));
+        gg = fmaf(P, A, Q * B);
+        hh = fmaf(P, B, -Q * A);
+    } else {
+        // clamped k_y, or the forward fallback of hc (diff_mod.f90:630-635): the bin
+        // (same phi, theta_i); m and evo2 = unit(k x m) in closed form
+        const float fk2c = kclamp ? copysignf(0.99999898672103882f, fk2) : fk2;
+        const float s2c = kclamp ? 2.02655679e-06f : s2;        // clamped: 1 - 0.999999^2
+        const float rs = abt_rsqf(s2c);
+        const float sq = s2c * rs;                                 // sqrt(1 - k2^2)
+        const float frac = sq * rf[F_SIG];
+        const float ifrac = rs * rf[F_ISIG];
+        const float da = -0.5f * fmaf(rf[F_KZ], frac, fk3 * ifrac);          // diff_ampl(1,1) = (2,2)
+        const float db = 0.5f * fmaf(rf[F_KXY], frac, -fk1 * fk2c * ifrac);  // diff_ampl(1,2) = -(2,1)
+        const float m1 = -fk1 * fk2c * rs, m2 = sq, m3 = -fk2c * fk3 * rs;
+        const float r13 = kclamp ? abt_rsqf(s2) : rs;
+        const float ev1 = -fk3 * r13, ev3 = fk1 * r13;
+        float hx = (float)uy, hy = (float)(-ux);
+        if (fwd) {
+            hx = (float)(bi.yt - r[R_COM + 1]);
+            hy = (float)(-(bi.xt - r[R_COM]));
+        }
+        const float rh = abt_rsqf(fmaf(hx, hx, hy * hy));
+        hx *= rh;
+        hy *= rh;
+        const float h1 = fmaf(rf[F_RR + 0], hx, rf[F_RR + 1] * hy);
+        const float h2 = fmaf(rf[F_RR + 3], hx, rf[F_RR + 4] * hy);
+        const float h3 = fmaf(rf[F_RR + 6], hx, rf[F_RR + 7] * hy);
+        const float p = fmaf(h1, m1, fmaf(h2, m2, h3 * m3));
+        const float qh = fmaf(h1, ev1, h3 * ev3);
+        gg = fmaf(p, da, qh * db);
+        hh = fmaf(p, db, -qh * da);
     }
-    const float rh = abt_rsqf(fmaf(hx, hx, hy * hy));
-    hx *= rh;
-    hy *= rh;
-    const float h1 = fmaf(rf[F_RR + 0], hx, rf[F_RR + 1] * hy);
-    const float h2 = fmaf(rf[F_RR + 3], hx, rf[F_RR + 4] * hy);
-    const float h3 = fmaf(rf[F_RR + 6], hx, rf[F_RR + 7] * hy);
-    const float p = fmaf(h1, m1, fmaf(h2, m2, h3 * m3));
-    const float qh = fmaf(h1, ev1, h3 * ev3);
-    // rot4 . diff_ampl = gg I + hh J, J = [[0,1],[-1,0]]
-    const float gg = fmaf(p, da, qh * db);
-    const float hh = fmaf(p, db, -qh * da);
     // f (gg I + hh J) = w P + w' Q with the projectors P, Q of J:  w = f (gg + i hh),
     // w' = f (gg - i hh);  P A = [r+; i r+],  Q A = [r-; -i r-]
     const float ta = fR * gg, tb = fI * hh, tc = fR * hh, td = fI * gg;
-    const double wr = (double)(ta - tb), wi = (double)(tc + td);
-    const double vr_ = (double)(ta + tb), vi = (double)(td - tc);
-    const double* rp = r + R_RP;
-    const double* rm = r + R_RM;
-    acc[0] = fma(wr, rp[0], fma(-wi, rp[1], acc[0]));
-    acc[1] = fma(wr, rp[1], fma(wi, rp[0], acc[1]));
-    acc[2] = fma(wr, rp[2], fma(-wi, rp[3], acc[2]));
-    acc[3] = fma(wr, rp[3], fma(wi, rp[2], acc[3]));
-    acc[4] = fma(vr_, rm[0], fma(-vi, rm[1], acc[4]));
-    acc[5] = fma(vr_, rm[1], fma(vi, rm[0], acc[5]));
-    acc[6] = fma(vr_, rm[2], fma(-vi, rm[3], acc[6]));
-    acc[7] = fma(vr_, rm[3], fma(vi, rm[2], acc[7]));
+    const float wr = ta - tb, wi = tc + td, vr_ = ta + tb, vi = td - tc;
+    const float* rp = rg + G_RP;
+    const float* rm = rg + G_RM;
+    accf[0] = fmaf(wr, rp[0], fmaf(-wi, rp[1], accf[0]));
+    accf[1] = fmaf(wr, rp[1], fmaf(wi, rp[0], accf[1]));
+    accf[2] = fmaf(wr, rp[2], fmaf(-wi, rp[3], accf[2]));
+    accf[3] = fmaf(wr, rp[3], fmaf(wi, rp[2], accf[3]));
+    accf[4] = fmaf(vr_, rm[0], fmaf(-vi, rm[1], accf[4]));
+    accf[5] = fmaf(vr_, rm[1], fmaf(vi, rm[0], accf[5]));
+    accf[6] = fmaf(vr_, rm[2], fmaf(-vi, rm[3], accf[6]));
+    accf[7] = fmaf(vr_, rm[3], fmaf(vi, rm[2], accf[7]));
 }
+
+// Beams per FP32 partial sum of the mixed kernels: every MIXED_GROUP consecutive beams
+// (counted from the start of the beam list, or of the orientation) are summed in FP32 and
+// the partial is added to the FP64 sums.  Tile sizes and chunk lengths are multiples of it,
+// so the groups, and the result, do not depend on them.
+constexpr int MIXED_GROUP = 8;
 
 // The per-bin phase factor e^{i k |x|} of the FP64 model body (adjbody.h).
 __host__ __device__ __forceinline__ void apply_bin_phase(double acc[8], double kr_rev)

@@ -181,16 +181,24 @@ __host__ __device__ inline void make_beam_record(const double* ampl, const doubl
     rec[R_AREA] = 0.5 * (rr.a[2][0] * ax + rr.a[2][1] * ay + rr.a[2][2] * az);
 
     // constants of the mixed-precision kernel
-    for (int j = 0; j < 3 && j < nv; j++) {
-        rec[R_HX + j] = 0.5 * rec[R_EX + j];
-        rec[R_HY + j] = 0.5 * rec[R_EY + j];
+    for (int j = 0; j < 3; j++) {
+        rec[R_KRR + j] = krev * rr.a[0][j];
+        rec[R_KRR + 3 + j] = krev * rr.a[1][j];
     }
+    rec[R_KP2] = krev * prop2[0];
+    rec[R_KP2 + 1] = krev * prop2[1];
     const double* A = rec + R_AMPL;   // sign-flipped amplitude matrix
     // r+ = (A(1,:) - i A(2,:))/2 and r- = (A(1,:) + i A(2,:))/2, columns 1 and 2
-    rec[R_RP + 0] = 0.5 * (A[0] + A[3]); rec[R_RP + 1] = 0.5 * (A[1] - A[2]);
-    rec[R_RP + 2] = 0.5 * (A[4] + A[7]); rec[R_RP + 3] = 0.5 * (A[5] - A[6]);
-    rec[R_RM + 0] = 0.5 * (A[0] - A[3]); rec[R_RM + 1] = 0.5 * (A[1] + A[2]);
-    rec[R_RM + 2] = 0.5 * (A[4] - A[7]); rec[R_RM + 3] = 0.5 * (A[5] + A[6]);
+    float* rg = reinterpret_cast<float*>(rec + R_G32);
+    rg[G_RP + 0] = (float)(0.5 * (A[0] + A[3])); rg[G_RP + 1] = (float)(0.5 * (A[1] - A[2]));
+    rg[G_RP + 2] = (float)(0.5 * (A[4] + A[7])); rg[G_RP + 3] = (float)(0.5 * (A[5] - A[6]));
+    rg[G_RM + 0] = (float)(0.5 * (A[0] - A[3])); rg[G_RM + 1] = (float)(0.5 * (A[1] + A[2]));
+    rg[G_RM + 2] = (float)(0.5 * (A[4] - A[7])); rg[G_RM + 3] = (float)(0.5 * (A[5] + A[6]));
+    // Karczewski coefficients over krev (the mixed body's krev-scaled polarisation)
+    const double isig = sqrt(rec[R_KY1]);
+    rg[G_C1] = (float)(-0.5 * prop2[2] * rec[R_FRAC] / krev);
+    rg[G_C2] = (float)(-0.5 * isig / krev);
+    rg[G_C3] = (float)(0.5 * prop2[0] * prop2[1] * rec[R_FRAC] / krev);
     float* rf = reinterpret_cast<float*>(rec + R_F32);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) rf[F_RR + 3 * i + j] = (float)rr.a[i][j];

@@ -35,11 +35,10 @@ enum : int {
     R_EDX = 52,     // [3] x_{j+1} - x_j
     R_EDY = 55,     // [3] y_{j+1} - y_j
     R_V20 = 58,     // [9] com-relative global vertices (x,y,z per vertex)
-    // mixed-precision kernel constants
-    R_HX = 68,      // [3] 0.5 * aperture-frame x of vertex j (half phases)
-    R_HY = 71,      // [3] 0.5 * aperture-frame y of vertex j
-    R_RP = 74,      // [4] r+ = (A(1,:) - i A(2,:)) / 2: re/im of columns 1, 2
-    R_RM = 78,      // [4] r- = (A(1,:) + i A(2,:)) / 2
+    // mixed-precision kernel constants (krev = k / 2 pi, revolutions per um)
+    R_KRR = 68,     // [6] krev * rows 1 and 2 of matmul(rot2, rot)
+    R_KP2 = 74,     // [2] krev * prop2_x, krev * prop2_y
+    R_G32 = 76,     // second FP32 block (float index G_* below), 6 doubles
     R_F32 = 82,     // FP32 block (float index F_* below), 12 doubles
     R_SIGMA = 94,   // -1 when the amplitude matrix was sign-flipped (anti-parallel), else +1
     REC = 96        // padded to a multiple of 4 doubles (32 bytes)
@@ -59,6 +58,16 @@ enum : int {
     F_END = 23
 };
 
+// float offsets inside the second FP32 block
+enum : int {
+    G_RP = 0,       // [4] r+ = (A(1,:) - i A(2,:)) / 2: re/im of columns 1, 2
+    G_RM = 4,       // [4] r- = (A(1,:) + i A(2,:)) / 2
+    G_C1 = 8,       // -prop2_z / (2 sqrt(1 - prop2_y^2) krev)
+    G_C2 = 9,       // -sqrt(1 - prop2_y^2) / (2 krev)
+    G_C3 = 10,      // prop2_x prop2_y / (2 sqrt(1 - prop2_y^2) krev)
+    G_END = 11
+};
+
 // ---- vertex record (nv > 3): VREC doubles per vertex -------------------------
 enum : int {
     V_X = 0, V_Y = 1, V_M = 2, V_N = 3, V_DX = 4, V_DY = 5, V_V20 = 6, VREC = 12
@@ -69,7 +78,9 @@ struct DevGrid {
     int32_t nphi, ntheta, nbins, theta_i;   // theta_i: 0-based forward-fallback column
     double lambda, waveno;                  // waveno = 2*pi_r4/lambda
     double krev;                            // waveno / (2 pi), revolutions per um
+    double krev_k2clamp;                    // krev * K2CLAMP (mixed kernel's k_y clamp test)
     float inv_lambda_f;                     // 1/lambda (FP32 body of the mixed kernel)
+    float inv_krev_f;                       // 1/krev
     const double* x;                        // [nbins] far-field bin positions
     const double* y;
     const double* z;

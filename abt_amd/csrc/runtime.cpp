@@ -248,7 +248,9 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     g->d.lambda = lambda;
     g->d.waveno = waveno;
     g->d.krev = krev;
+    g->d.krev_k2clamp = krev * abt::K2CLAMP;
     g->d.inv_lambda_f = (float)(1.0 / lambda);
+    g->d.inv_krev_f = (float)(1.0 / krev);
     g->d.x = d;
     g->d.y = d + nb;
     g->d.z = d + 2 * nb;

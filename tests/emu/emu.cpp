@@ -20,7 +20,9 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
     g.lambda = lambda;
     g.waveno = 2.0 * PI_R4 / lambda;
     g.krev = g.waveno / 6.283185307179586476925;
+    g.krev_k2clamp = g.krev * K2CLAMP;
     g.inv_lambda_f = (float)(1.0 / lambda);
+    g.inv_krev_f = (float)(1.0 / g.krev);
     const int nbins = g.nbins;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int bin = 0; bin < nbins; bin++) {
@@ -29,6 +31,7 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
         const double rad = sqrt(x[bin] * x[bin] + y[bin] * y[bin] + z[bin] * z[bin]);
         BinIn bi{x[bin], y[bin], z[bin], x[bt], y[bt], z[bt], rad};
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        float accf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int64_t b = 0; b < nb; b++) {
             if (prec == 0)
                 beam_bin_fp64(recs + b * REC, vrecs, g, bi, cphi[iphi], sphi[iphi], fast_diff, acc);
@@ -37,8 +40,14 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
                 adj_beam_from_record(recs + b * REC, B);
                 model_bin_fp64(B, bi.x, bi.y, bi.z, bi.xt, bi.yt, rad, g.waveno, g.lambda, acc);
             }
-            else
-                beam_bin_mixed(recs + b * REC, vrecs, g, bi, fast_diff, acc);
+            else {   // FP32 sums over groups of MIXED_GROUP beams, as the kernels form them
+                beam_bin_mixed(recs + b * REC, vrecs, g, bi, fast_diff, accf);
+                if ((b + 1) % MIXED_GROUP == 0 || b + 1 == nb)
+                    for (int q = 0; q < 8; q++) {
+                        acc[q] += (double)accf[q];
+                        accf[q] = 0.f;
+                    }
+            }
         }
         if (prec == 1) mixed_finish(acc, cphi[iphi], sphi[iphi]);
         if (prec == 3) model_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
