@@ -125,6 +125,54 @@ def far_field(system: FarFieldSystem, verts_b, ampl_b, verts_e, ampl_e):
     return _FarField.apply(system, verts_b, ampl_b, verts_e, ampl_e)
 
 
+class _GatherRows(torch.autograd.Function):
+    """V.index_select(0, vid) with a backward that sums each row of V's gradient from a
+    padded table of the rows that gathered it: one gather and a dense sum, no atomics and
+    no sort per call, so the result does not depend on the order of the additions' threads."""
+
+    @staticmethod
+    def forward(ctx, V, vid, table):
+        ctx.save_for_backward(table)
+        return V.index_select(0, vid)
+
+    @staticmethod
+    def backward(ctx, g):
+        table, = ctx.saved_tensors
+        gz = torch.cat([g, g.new_zeros((1,) + tuple(g.shape[1:]))])
+        return gz[table].sum(1), None, None
+
+
+class ApertureGather:
+    """Aperture vertices of every beam, R_o V[faces[fi - 1]], shape (n, 3, 3), as a
+    differentiable function of the particle vertices V (nv, 3).  faces: (nfaces, 3) vertex
+    indices; fi: 1-based facet id per beam (the reference's outbeamtype%fi, types_mod.f90:8-17,
+    whose vertices diff_mod.f90:529-531 looks up); rot: (norient, 3, 3); orient: orientation
+    index per beam.  The facet ids are fixed at construction, where the backward's row table
+    (rows of each vertex, padded with a zero row) is built once."""
+
+    def __init__(self, faces, fi, nv):
+        vid = faces[fi - 1].reshape(-1)
+        n = vid.numel()
+        order = torch.argsort(vid, stable=True)
+        counts = torch.bincount(vid, minlength=nv)
+        start = torch.cumsum(counts, 0) - counts
+        sv = vid[order]
+        pos = torch.arange(n, device=vid.device) - start[sv]
+        width = max(int(counts.max()) if n else 0, 1)
+        table = torch.full((nv, width), n, dtype=torch.long, device=vid.device)
+        table[sv, pos] = order
+        self.vid, self.table = vid, table
+
+    def __call__(self, V, rot, orient):
+        P = _GatherRows.apply(V, self.vid, self.table).reshape(-1, 3, 3)
+        return torch.einsum("bij,bvj->bvi", rot[orient], P)
+
+
+def aperture_vertices(V, faces, fi, rot, orient):
+    """One-off form of ApertureGather (builds the row table on every call)."""
+    return ApertureGather(faces, fi, V.shape[0])(V, rot, orient)
+
+
 def mueller_11_12(X):
     """Orientation-averaged S11 and S12 per bin (ampl_to_mueller, outputs_mod.f90:489-543,
     then the mean over orientations, divide_by_num_orientations outputs_mod.f90:13-43)."""

@@ -53,6 +53,7 @@ struct AdjBeam {
     double P[3], Kz;            // propagation vector, P.e_z
     double C[8];                // sigma * ampl
     double xl[3], yl[3];        // in-plane coordinates of v_l - c
+    double xr[3], yr[3];        // the same over 2 pi (mixed body: phases in revolutions)
     double area;                // |N| / 2
     double qa2;                 // (aperture radius)^2: series branch when |q|^2 qa2 < SERIES_QA^2
     // FP32 copies for the mixed-precision body
@@ -79,6 +80,8 @@ __host__ __device__ inline void adj_beam_from_record(const double* __restrict__ 
         B.P[i] = r[R_PROP0 + i];
         B.xl[i] = r[R_EX + i];
         B.yl[i] = r[R_EY + i];
+        B.xr[i] = B.xl[i] * 0.15915494309189535;
+        B.yr[i] = B.yl[i] * 0.15915494309189535;
     }
     B.cc = r[R_CC];
     B.Kz = B.P[0] * B.ez[0] + B.P[1] * B.ez[1] + B.P[2] * B.ez[2];
@@ -98,6 +101,38 @@ __host__ __device__ inline void adj_beam_from_record(const double* __restrict__ 
     const double a2 = (B.xl[1] - B.xl[0]) * (B.yl[2] - B.yl[0]) - (B.xl[2] - B.xl[0]) * (B.yl[1] - B.yl[0]);
     B.fso = a2 < 0.0 ? 1.0f : -1.0f;
 }
+
+// FP32 partial sums of the mixed body over a group of bins, added into AdjAcc
+struct AdjAccF {
+    float GC[8], S0, Qx[3], Qy[3], cb[3], exb[3], eyb[3], ezb[3];
+};
+
+__host__ __device__ inline void adj_zero_f(AdjAccF& a)
+{
+    for (int q = 0; q < 8; q++) a.GC[q] = 0.f;
+    a.S0 = 0.f;
+    for (int i = 0; i < 3; i++) {
+        a.Qx[i] = a.Qy[i] = a.cb[i] = 0.f;
+        a.exb[i] = a.eyb[i] = a.ezb[i] = 0.f;
+    }
+}
+
+__host__ __device__ inline void adj_add(AdjAcc& a, const AdjAccF& f)
+{
+    for (int q = 0; q < 8; q++) a.GC[q] += (double)f.GC[q];
+    a.S0 += (double)f.S0;
+    for (int i = 0; i < 3; i++) {
+        a.Qx[i] += (double)f.Qx[i];
+        a.Qy[i] += (double)f.Qy[i];
+        a.cb[i] += (double)f.cb[i];
+        a.exb[i] += (double)f.exb[i];
+        a.eyb[i] += (double)f.eyb[i];
+        a.ezb[i] += (double)f.ezb[i];
+    }
+}
+
+// bins per FP32 partial sum of the mixed adjoint
+constexpr int ADJ_GROUP = 8;
 
 __host__ __device__ inline void adj_zero(AdjAcc& a)
 {
@@ -443,8 +478,12 @@ __host__ __device__ inline void model_finish(double acc[8], double cp, double sp
 // FP32 series of the form factor and its gradient.  Edge phases come from the three vertex half-phase
 // exponentials w_l = e^{i alpha_l / 2}: e^{i mu_j} = w_j w_{j+1},
 // e^{i K_j / 2} = w_{j+1} conj(w_j); sinc(K_j / 2) from its series for |K_j| < 1.
-__host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn, const double* __restrict__ lam,
-                                              double waveno, double lambda, AdjAcc& acc)
+// Mixed body: FP64 geometry, q and phases; the far-field phase e^{i k rho} is folded into
+// the vertex exponentials, so F and dF/dq below carry it (F' = e^{i k rho} F); FP32
+// elsewhere; contributions go to FP32 partial sums (the caller adds them into FP64 every
+// ADJ_GROUP bins).  lam: Lam' of this bin in FP32.
+__host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn, const float* __restrict__ lam,
+                                              double waveno, double lambda, AdjAccF& acc)
 {
     const double k = waveno;
     const double dx = bn.x - B.c[0], dy = bn.y - B.c[1], dz = bn.z - B.c[2];
@@ -454,19 +493,9 @@ __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn
     y0 = fma(0.5 * y0, e, y0);
     e = fma(-s2 * y0, y0, 1.0);
     const double ir = fma(0.5 * y0, e, y0);              // 1 / rho
-    const double rho = s2 * ir;
     const double kh[3] = {dx * ir, dy * ir, dz * ir};
-    // e^{i k rho}: k (rho - |x|) = k (|c|^2 - 2 x.c) / (rho + |x|), plus k |x|
-    const double rc = fma(bn.x, B.c[0], fma(bn.y, B.c[1], bn.z * B.c[2]));
-    const double den = rho + bn.rad;
-    double rd = (double)abt_rcpf((float)den);
-    rd = fma(fma(-den, rd, 1.0), rd, rd);
-    rd = fma(fma(-den, rd, 1.0), rd, rd);
-    const double drho = (B.cc - 2.0 * rc) * rd;
     const double krev = k * 0.15915494309189535;
-    unsigned sg;
-    const float rE = half_reduce((bn.kr_rev - rint(bn.kr_rev)) + krev * drho, &sg);
-    const float Es = flipsign(rE * sin_over_r(rE * rE), sg), Ec = flipsign(cos_poly(rE * rE), sg);
+    const double kd = krev * (s2 * ir);                   // k rho / 2 pi
     const double q3[3] = {k * (B.P[0] - kh[0]), k * (B.P[1] - kh[1]), k * (B.P[2] - kh[2])};
     const double qx = fma(q3[0], B.ex[0], fma(q3[1], B.ex[1], q3[2] * B.ex[2]));
     const double qy = fma(q3[0], B.ey[0], fma(q3[1], B.ey[1], q3[2] * B.ey[2]));
@@ -497,29 +526,34 @@ __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn
             hm3 = hm2; hm2 = hm1; hm1 = hn; hprev = hn;
         }
         const float A2 = (float)(2.0 * B.area);
-        F[0] = A2 * fr;
-        F[1] = A2 * fi;
-        Gx[0] = A2 * (dr[0] * B.fxl[0] + dr[1] * B.fxl[1] + dr[2] * B.fxl[2]);
-        Gx[1] = A2 * (di[0] * B.fxl[0] + di[1] * B.fxl[1] + di[2] * B.fxl[2]);
-        Gy[0] = A2 * (dr[0] * B.fyl[0] + dr[1] * B.fyl[1] + dr[2] * B.fyl[2]);
-        Gy[1] = A2 * (di[0] * B.fyl[0] + di[1] * B.fyl[1] + di[2] * B.fyl[2]);
+        // times the far-field phase e^{i k rho}
+        float Ec, Es;
+        expi_pi(2.0 * kd, &Ec, &Es);
+        const float Er = A2 * Ec, Ei = A2 * Es;
+        const float gxr = dr[0] * B.fxl[0] + dr[1] * B.fxl[1] + dr[2] * B.fxl[2];
+        const float gxi = di[0] * B.fxl[0] + di[1] * B.fxl[1] + di[2] * B.fxl[2];
+        const float gyr = dr[0] * B.fyl[0] + dr[1] * B.fyl[1] + dr[2] * B.fyl[2];
+        const float gyi = di[0] * B.fyl[0] + di[1] * B.fyl[1] + di[2] * B.fyl[2];
+        F[0] = Er * fr - Ei * fi;
+        F[1] = Er * fi + Ei * fr;
+        Gx[0] = Er * gxr - Ei * gxi;
+        Gx[1] = Er * gxi + Ei * gxr;
+        Gy[0] = Er * gyr - Ei * gyi;
+        Gy[1] = Er * gyi + Ei * gyr;
     } else {
-        // vertex half phases alpha_l / 2 in revolutions: (q . u_l) / (4 pi)
+        // vertex half phases alpha_l / 2 plus k rho / 2: w_l = e^{i pi t_l}, t_l = alpha_l / 2 pi + k rho / 2 pi
         float wr[3], wi[3];
         double al[3];
         for (int l = 0; l < 3; l++) {
-            al[l] = fma(qx, B.xl[l], qy * B.yl[l]);
-            unsigned sl;
-            const float rl = half_reduce(al[l] * 0.07957747154594767, &sl);
-            wi[l] = flipsign(rl * sin_over_r(rl * rl), sl);
-            wr[l] = flipsign(cos_poly(rl * rl), sl);
+            al[l] = fma(qx, B.xr[l], fma(qy, B.yr[l], kd));
+            expi_pi(al[l], &wr[l], &wi[l]);
         }
         float sr = 0.f, si = 0.f, gxr = 0.f, gxi = 0.f, gyr = 0.f, gyi = 0.f;
         for (int j = 0; j < 3; j++) {
             const int jn = j == 2 ? 0 : j + 1;
             const float ex = B.fxl[jn] - B.fxl[j], ey = B.fyl[jn] - B.fyl[j];
             const float mx = 0.5f * (B.fxl[j] + B.fxl[jn]), my = 0.5f * (B.fyl[j] + B.fyl[jn]);
-            const float x = (float)(0.5 * (al[jn] - al[j]));          // K_j / 2
+            const float x = 3.14159265358979f * (float)(al[jn] - al[j]);   // K_j / 2
             // e^{i x} = w_{j+1} conj(w_j), e^{i mu} = w_j w_{j+1}
             const float cx = wr[jn] * wr[j] + wi[jn] * wi[j];
             const float sx = wi[jn] * wr[j] - wr[jn] * wi[j];
@@ -556,43 +590,64 @@ __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn
     const float k1 = fmaf(kf[0], B.fex[0], fmaf(kf[1], B.fex[1], kf[2] * B.fex[2]));
     const float k3 = fmaf(kf[0], B.fez[0], fmaf(kf[1], B.fez[1], kf[2] * B.fez[2]));
     const float k2 = (float)k2d;
-    const float k2c = clamp ? copysignf(0.99999898672103882f, k2) : k2;
     const float s13q = fmaf(k1, k1, k3 * k3);                    // 1 - k2^2
-    const float r13 = abt_rsqf(s13q);
-    const float s13 = s13q * r13;
-    const float sqq = clamp ? 2.02655679e-06f : s13q;
-    const float rsq = clamp ? abt_rsqf(sqq) : r13;
-    const float sq = sqq * rsq;
     const bool fb = !(fabs(kh[2]) < FWD_DOT);
-    float hvx, hvy;
-    if (fb) {
-        hvx = (float)(bn.yt - B.c[1]);
-        hvy = (float)(-(bn.xt - B.c[0]));
+    // Away from the clamp and the forward fallback, hc = unit(inc x khat) is normal to
+    // khat, so hc.m = D' P0 and hc.unit(k x m) = D' Q0 with D' = rh / sqrt(1 - k2^2),
+    // rh = 1 / |inc x khat|, P0 = e_y . (kh_y, -kh_x, 0), Q0 = e_y . ((inc.khat) khat - inc),
+    // and g = D (P0 A + Q0 B), h = D (P0 B - Q0 A) with D = rh / (1 - k2^2),
+    // A = -(K_z (1 - k2^2) + k3) / 2, B = -k1 k2 / 2.  P0, Q0 in FP64: both vanish at khat = +-e_y.
+    const bool fast = !clamp && !fb;
+    float g, h;
+    float fP0 = 0.f, fQ0 = 0.f, fA = 0.f, fB = 0.f, fD = 0.f, rh = 0.f, is2 = 0.f;
+    float k2c = 0.f, r13 = 0.f, s13 = 0.f, rsq = 0.f, sq = 0.f, inh = 0.f, hc0 = 0.f, hc1 = 0.f;
+    float mv[3] = {0.f, 0.f, 0.f}, ev[3] = {0.f, 0.f, 0.f}, p = 0.f, qh = 0.f, da = 0.f, db = 0.f;
+    if (fast) {
+        fP0 = (float)fma(B.ey[0], kh[1], -B.ey[1] * kh[0]);
+        fQ0 = (float)fma(-kh[2], k2d, B.ey[2]);
+        rh = abt_rsqf(fmaf(kf[0], kf[0], kf[1] * kf[1]));
+        is2 = abt_rcpf(s13q);
+        fD = rh * is2;
+        fA = -0.5f * fmaf(B.fKz, s13q, k3);
+        fB = -0.5f * k1 * k2;
+        g = fD * fmaf(fP0, fA, fQ0 * fB);
+        h = fD * fmaf(fP0, fB, -fQ0 * fA);
     } else {
-        hvx = kf[1];
-        hvy = -kf[0];
+        k2c = clamp ? copysignf(0.99999898672103882f, k2) : k2;
+        r13 = abt_rsqf(s13q);
+        s13 = s13q * r13;
+        const float sqq = clamp ? 2.02655679e-06f : s13q;
+        rsq = clamp ? abt_rsqf(sqq) : r13;
+        sq = sqq * rsq;
+        float hvx, hvy;
+        if (fb) {
+            hvx = (float)(bn.yt - B.c[1]);
+            hvy = (float)(-(bn.xt - B.c[0]));
+        } else {
+            hvx = kf[1];
+            hvy = -kf[0];
+        }
+        inh = abt_rsqf(fmaf(hvx, hvx, hvy * hvy));
+        hc0 = hvx * inh;
+        hc1 = hvy * inh;
+        const float uu[3] = {kf[1] * B.fey[2] - kf[2] * B.fey[1], kf[2] * B.fey[0] - kf[0] * B.fey[2],
+                             kf[0] * B.fey[1] - kf[1] * B.fey[0]};
+        for (int i = 0; i < 3; i++) {
+            mv[i] = (B.fey[i] - k2c * kf[i]) * rsq;
+            ev[i] = uu[i] * r13;
+        }
+        p = fmaf(hc0, mv[0], hc1 * mv[1]);
+        qh = fmaf(hc0, ev[0], hc1 * ev[1]);
+        da = -0.5f * fmaf(B.fKz, sq, k3 * rsq);
+        db = -0.5f * k1 * k2c * rsq;
+        g = fmaf(p, da, qh * db);
+        h = fmaf(p, db, -qh * da);
     }
-    const float inh = abt_rsqf(fmaf(hvx, hvx, hvy * hvy));
-    const float hc0 = hvx * inh, hc1 = hvy * inh;
-    float mv[3], ev[3];
-    float uu[3] = {kf[1] * B.fey[2] - kf[2] * B.fey[1], kf[2] * B.fey[0] - kf[0] * B.fey[2],
-                   kf[0] * B.fey[1] - kf[1] * B.fey[0]};
-    for (int i = 0; i < 3; i++) {
-        mv[i] = (B.fey[i] - k2c * kf[i]) * rsq;
-        ev[i] = uu[i] * r13;
-    }
-    const float p = fmaf(hc0, mv[0], hc1 * mv[1]);
-    const float qh = fmaf(hc0, ev[0], hc1 * ev[1]);
-    const float da = -0.5f * fmaf(B.fKz, sq, k3 * rsq);
-    const float db = -0.5f * k1 * k2c * rsq;
-    const float g = fmaf(p, da, qh * db);
-    const float h = fmaf(p, db, -qh * da);
-    // s = E (i / lambda) F
+    // s = E (i / lambda) F  (F carries E)
     const float il = (float)(1.0 / lambda);
-    const float EFr = Ec * F[0] - Es * F[1], EFi = Ec * F[1] + Es * F[0];
-    const float sr = -EFi * il, si = EFr * il;
+    const float sr = -F[1] * il, si = F[0] * il;
     float L[8];
-    for (int q = 0; q < 8; q++) L[q] = (float)lam[q];
+    for (int q = 0; q < 8; q++) L[q] = lam[q];
     float z1r = 0.f, z1i = 0.f, z2r = 0.f, z2i = 0.f;
     for (int cc = 0; cc < 2; cc++) {
         const int i1 = 2 * (0 + 2 * cc), i2 = 2 * (1 + 2 * cc);
@@ -607,78 +662,97 @@ __host__ __device__ inline void adj_bin_mixed(const AdjBeam& B, const AdjBin& bn
         const int i1 = 2 * (0 + 2 * cc), i2 = 2 * (1 + 2 * cc);
         const float r1r = g * L[i1] - h * L[i2], r1i = g * L[i1 + 1] - h * L[i2 + 1];
         const float r2r = g * L[i2] + h * L[i1], r2i = g * L[i2 + 1] + h * L[i1 + 1];
-        acc.GC[i1] += (double)(sr * r1r + si * r1i);
-        acc.GC[i1 + 1] += (double)(sr * r1i - si * r1r);
-        acc.GC[i2] += (double)(sr * r2r + si * r2i);
-        acc.GC[i2 + 1] += (double)(sr * r2i - si * r2r);
+        acc.GC[i1] += sr * r1r + si * r1i;
+        acc.GC[i1 + 1] += sr * r1i - si * r1r;
+        acc.GC[i2] += sr * r2r + si * r2i;
+        acc.GC[i2 + 1] += sr * r2i - si * r2r;
     }
     const float gg = sr * z1r - si * z1i, gh = sr * z2r - si * z2i;
     const float mur = g * z1r + h * z2r, mui = g * z1i + h * z2i;
-    const float mEr = mur * Ec - mui * Es, mEi = mur * Es + mui * Ec;
-    const float nur = -mEi * il, nui = mEr * il;
+    // nu = mu E (i / lambda); E is in F and dF/dq
+    const float nur = -mui * il, nui = mur * il;
     const float nFr = nur * F[0] - nui * F[1], nFi = nur * F[1] + nui * F[0];
     const float Rx = nur * Gx[0] - nui * Gx[1];
     const float Ry = nur * Gy[0] - nui * Gy[1];
-    acc.S0 += (double)nFr;
-    const double dRx = (double)Rx, dRy = (double)Ry;
+    acc.S0 += nFr;
     for (int i = 0; i < 3; i++) {
-        acc.Qx[i] = fma(q3[i], dRx, acc.Qx[i]);
-        acc.Qy[i] = fma(q3[i], dRy, acc.Qy[i]);
+        const float fq = (float)q3[i];
+        acc.Qx[i] = fmaf(fq, Rx, acc.Qx[i]);
+        acc.Qy[i] = fmaf(fq, Ry, acc.Qy[i]);
     }
     const float kw = (float)k;
     float kb[3];
     for (int i = 0; i < 3; i++) kb[i] = -kw * (Rx * B.fex[i] + Ry * B.fey[i]);
-    const float pb = gg * da + gh * db, qb = gg * db - gh * da;
-    const float dab = gg * p - gh * qh, dbb = gg * qh + gh * p;
-    const float Kzb = -0.5f * sq * dab;
-    float sqb = -0.5f * (B.fKz - k3 * rsq * rsq) * dab + 0.5f * k1 * k2c * rsq * rsq * dbb;
-    const float k3b = -0.5f * dab * rsq;
-    const float k1b = -0.5f * k2c * rsq * dbb;
-    float k2cb = -0.5f * k1 * rsq * dbb;
-    const float hcb0 = pb * mv[0] + qb * ev[0], hcb1 = pb * mv[1] + qb * ev[1];
-    const float mb[3] = {pb * hc0, pb * hc1, 0.f};
-    k2cb += -(mb[0] * kf[0] + mb[1] * kf[1]) * rsq;
-    sqb += -(mb[0] * mv[0] + mb[1] * mv[1]) * rsq;
-    const float ub[3] = {qb * hc0 * r13, qb * hc1 * r13, 0.f};
-    const float s13b = -qb * (hc0 * ev[0] + hc1 * ev[1]) * r13;
-    float eyb[3], t3[3];
-    for (int i = 0; i < 3; i++) {
-        eyb[i] = mb[i] * rsq;
-        kb[i] += -k2c * rsq * mb[i];
+    float k1b, k2b, k3b, Kzb, eyb[3];
+    float cbf[3] = {0.f, 0.f, 0.f};
+    if (fast) {
+        // reverse of g, h: the 1 - k2^2 inside A and D differentiated as a function of k2
+        const float Pb = fD * fmaf(gg, fA, gh * fB);
+        const float Qb = fD * fmaf(gg, fB, -gh * fA);
+        const float Ab = fD * fmaf(gg, fP0, -gh * fQ0);
+        const float Bb = fD * fmaf(gg, fQ0, gh * fP0);
+        const float w = fmaf(gg, g, gh * h);                 // (dL/dD) D
+        const float rw = w * rh * rh;
+        Kzb = -0.5f * s13q * Ab;
+        k3b = -0.5f * Ab;
+        k1b = -0.5f * k2 * Bb;
+        k2b = B.fKz * k2 * Ab - 0.5f * k1 * Bb + 2.f * k2 * w * is2 - kf[2] * Qb;
+        kb[0] += -rw * kf[0] - B.fey[1] * Pb;
+        kb[1] += -rw * kf[1] + B.fey[0] * Pb;
+        kb[2] += -k2 * Qb;
+        eyb[0] = kf[1] * Pb;
+        eyb[1] = -kf[0] * Pb;
+        eyb[2] = Qb;
+    } else {
+        const float pb = gg * da + gh * db, qb = gg * db - gh * da;
+        const float dab = gg * p - gh * qh, dbb = gg * qh + gh * p;
+        Kzb = -0.5f * sq * dab;
+        float sqb = -0.5f * (B.fKz - k3 * rsq * rsq) * dab + 0.5f * k1 * k2c * rsq * rsq * dbb;
+        k3b = -0.5f * dab * rsq;
+        k1b = -0.5f * k2c * rsq * dbb;
+        float k2cb = -0.5f * k1 * rsq * dbb;
+        const float hcb0 = pb * mv[0] + qb * ev[0], hcb1 = pb * mv[1] + qb * ev[1];
+        const float mb[3] = {pb * hc0, pb * hc1, 0.f};
+        k2cb += -(mb[0] * kf[0] + mb[1] * kf[1]) * rsq;
+        sqb += -(mb[0] * mv[0] + mb[1] * mv[1]) * rsq;
+        const float ub[3] = {qb * hc0 * r13, qb * hc1 * r13, 0.f};
+        const float s13b = -qb * (hc0 * ev[0] + hc1 * ev[1]) * r13;
+        for (int i = 0; i < 3; i++) {
+            eyb[i] = mb[i] * rsq;
+            kb[i] += -k2c * rsq * mb[i];
+        }
+        // kb += e_y x ub; eyb += ub x khat  (ub has no z component)
+        kb[0] += -B.fey[2] * ub[1];
+        kb[1] += B.fey[2] * ub[0];
+        kb[2] += B.fey[0] * ub[1] - B.fey[1] * ub[0];
+        eyb[0] += ub[1] * kf[2];
+        eyb[1] += -ub[0] * kf[2];
+        eyb[2] += ub[0] * kf[1] - ub[1] * kf[0];
+        k2b = -k2 * r13 * s13b;
+        k2cb += -k2c * rsq * sqb;
+        if (!clamp) k2b += k2cb;
+        (void)s13;
+        const float hdot = hcb0 * hc0 + hcb1 * hc1;
+        const float hvbx = (hcb0 - hdot * hc0) * inh, hvby = (hcb1 - hdot * hc1) * inh;
+        const float tx = -hvby, ty = hvbx;
+        if (fb) {
+            cbf[0] = -tx;
+            cbf[1] = -ty;
+        } else {
+            kb[0] += tx;
+            kb[1] += ty;
+        }
     }
-    // kb += e_y x ub; eyb += ub x khat  (ub has no z component)
-    t3[0] = -B.fey[2] * ub[1];
-    t3[1] = B.fey[2] * ub[0];
-    t3[2] = B.fey[0] * ub[1] - B.fey[1] * ub[0];
-    for (int i = 0; i < 3; i++) kb[i] += t3[i];
-    eyb[0] += ub[1] * kf[2];
-    eyb[1] += -ub[0] * kf[2];
-    eyb[2] += ub[0] * kf[1] - ub[1] * kf[0];
-    float k2b = -k2 * r13 * s13b;
-    k2cb += -k2c * rsq * sqb;
-    if (!clamp) k2b += k2cb;
-    (void)s13;
     for (int i = 0; i < 3; i++) {
         kb[i] += k1b * B.fex[i] + k2b * B.fey[i] + k3b * B.fez[i];
-        acc.exb[i] += (double)(k1b * kf[i]);
-        acc.eyb[i] += (double)(eyb[i] + k2b * kf[i]);
-        acc.ezb[i] += (double)(k3b * kf[i] + Kzb * B.fP[i]);
-    }
-    const float hdot = hcb0 * hc0 + hcb1 * hc1;
-    const float hvbx = (hcb0 - hdot * hc0) * inh, hvby = (hcb1 - hdot * hc1) * inh;
-    const float tx = -hvby, ty = hvbx;
-    float cbf[3] = {0.f, 0.f, 0.f};
-    if (fb) {
-        cbf[0] = -tx;
-        cbf[1] = -ty;
-    } else {
-        kb[0] += tx;
-        kb[1] += ty;
+        acc.exb[i] += k1b * kf[i];
+        acc.eyb[i] += eyb[i] + k2b * kf[i];
+        acc.ezb[i] += k3b * kf[i] + Kzb * B.fP[i];
     }
     const float kk = kb[0] * kf[0] + kb[1] * kf[1] + kb[2] * kf[2];
     const float irf = (float)ir;
     const float ph = kw * nFi;
-    for (int i = 0; i < 3; i++) acc.cb[i] += (double)(cbf[i] - (kb[i] - kk * kf[i]) * irf + ph * kf[i]);
+    for (int i = 0; i < 3; i++) acc.cb[i] += cbf[i] - (kb[i] - kk * kf[i]) * irf + ph * kf[i];
 }
 
 // Per-beam gradients from the accumulated sums: gv[9] = dL/dv_l (x,y,z per vertex, the

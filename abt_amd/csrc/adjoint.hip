@@ -24,11 +24,13 @@ namespace abt {
 
 constexpr int AL = 64;   // lanes per workgroup (one wavefront)
 
-// Lam' = k G T_phi^T per (orientation, bin), [o][bin][8]; G in the batched forward's
-// amplitude layout [o][8][nbins].  T_phi = [[-s, -c], [c, -s]] (diff_mod.f90:646-650).
+// Lam' = k G T_phi^T per (orientation, bin), [o][bin][8] (FP64, or FP32 for the mixed
+// kernel); G in the batched forward's amplitude layout [o][8][nbins].
+// T_phi = [[-s, -c], [c, -s]] (diff_mod.f90:646-650).
+template <typename T>
 __global__ __launch_bounds__(256) void k_adj_lam(const double* __restrict__ G, const double* __restrict__ cphi,
                                                  const double* __restrict__ sphi, int nphi, int nbins,
-                                                 int norient, double k, double* __restrict__ lam)
+                                                 int norient, double k, T* __restrict__ lam)
 {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= (int64_t)norient * nbins) return;
@@ -39,15 +41,15 @@ __global__ __launch_bounds__(256) void k_adj_lam(const double* __restrict__ G, c
     double L[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) L[q] = k * g[(int64_t)q * nbins];
-    double* out = lam + i * 8;
+    T* out = lam + i * 8;
     // pair (r, col): index 2 (r + 2 col)
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
         for (int ri = 0; ri < 2; ri++) {
             const double l1 = L[2 * r + ri], l2 = L[2 * (r + 2) + ri];
-            out[2 * r + ri] = -s * l1 - c * l2;            // Lam'(r,1)
-            out[2 * (r + 2) + ri] = c * l1 - s * l2;       // Lam'(r,2)
+            out[2 * r + ri] = (T)(-s * l1 - c * l2);            // Lam'(r,1)
+            out[2 * (r + 2) + ri] = (T)(c * l1 - s * l2);       // Lam'(r,2)
         }
 }
 
@@ -81,14 +83,27 @@ __global__ __launch_bounds__(AL, 2) void k_adjoint(AdjArgs a)
     const int chunk_bins = (a.nbins + a.nchunks - 1) / a.nchunks;
     const int b0 = U.chunk * chunk_bins;
     const int b1 = min(a.nbins, b0 + chunk_bins);
-    const double* lam = a.lam + (int64_t)U.o * a.nbins * 8;
-    for (int i = b0; i < b1; i++) {
-        const double* bb = a.bins + (int64_t)i * 8;
-        const AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
-        if (PREC == 0)
+    if (PREC == 0) {
+        const double* lam = a.lam + (int64_t)U.o * a.nbins * 8;
+        for (int i = b0; i < b1; i++) {
+            const double* bb = a.bins + (int64_t)i * 8;
+            const AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
             adj_bin_fp64(B, bn, lam + (int64_t)i * 8, a.waveno, a.lambda, acc);
-        else
-            adj_bin_mixed(B, bn, lam + (int64_t)i * 8, a.waveno, a.lambda, acc);
+        }
+    } else {
+        // FP32 partial sums over groups of ADJ_GROUP bins, added into FP64
+        const float* lam = reinterpret_cast<const float*>(a.lam) + (int64_t)U.o * a.nbins * 8;
+        for (int i0 = b0; i0 < b1; i0 += ADJ_GROUP) {
+            AdjAccF af;
+            adj_zero_f(af);
+            const int i1 = min(b1, i0 + ADJ_GROUP);
+            for (int i = i0; i < i1; i++) {
+                const double* bb = a.bins + (int64_t)i * 8;
+                const AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
+                adj_bin_mixed(B, bn, lam + (int64_t)i * 8, a.waveno, a.lambda, af);
+            }
+            adj_add(acc, af);
+        }
     }
     if (!act) return;
     double gv[9], ga[8];
@@ -145,8 +160,12 @@ extern "C" int abt_launch_adjoint(const AdjArgs* a, int prec, int nunits, const 
                                   hipStream_t st)
 {
     const int64_t nl = (int64_t)norient * g->nbins;
-    hipLaunchKernelGGL(k_adj_lam, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, G, g->cphi, g->sphi,
-                       g->nphi, g->nbins, norient, g->waveno, a->lam);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_adj_lam<double>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, G, g->cphi,
+                           g->sphi, g->nphi, g->nbins, norient, g->waveno, a->lam);
+    else
+        hipLaunchKernelGGL(k_adj_lam<float>, dim3((unsigned)((nl + 255) / 256)), dim3(256), 0, st, G, g->cphi,
+                           g->sphi, g->nphi, g->nbins, norient, g->waveno, reinterpret_cast<float*>(a->lam));
     if (nunits > 0) {
         if (prec == 0)
             hipLaunchKernelGGL(k_adjoint<0>, dim3(nunits), dim3(AL), 0, st, *a);

@@ -390,9 +390,10 @@ def adjoint_bench(norient, lam, dev):
         ab = t["beams"]["ampl"].clone().requires_grad_(True)
         ae = t["ext"]["ampl"].clone().requires_grad_(True)
 
+        gathers = {tree: AG.ApertureGather(facesT, t[tree]["fi"], V.shape[0]) for tree in ("beams", "ext")}
+
         def apert(tree):
-            tr = t[tree]
-            return torch.einsum("bij,bvj->bvi", R[tr["orient"]], V[facesT[tr["fi"] - 1]])
+            return gathers[tree](V, R, t[tree]["orient"])
 
         times = {"forward": [], "backward": [], "forward_kernel": [], "adjoint_kernel": []}
         for rep in range(3):

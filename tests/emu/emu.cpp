@@ -95,10 +95,18 @@ extern "C" void emu_adjoint_mixed(const double* recs, int64_t nb, const double* 
         adj_beam_from_record(r, B);
         AdjAcc acc;
         adj_zero(acc);
+        AdjAccF af;
+        adj_zero_f(af);
         for (int64_t i = 0; i < nbins; i++) {
             const double* bb = bins + 7 * i;
             AdjBin bn{bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], bb[6]};
-            adj_bin_mixed(B, bn, lam + 8 * i, waveno, lambda, acc);
+            float lf[8];
+            for (int q = 0; q < 8; q++) lf[q] = (float)lam[8 * i + q];
+            adj_bin_mixed(B, bn, lf, waveno, lambda, af);
+            if ((i + 1) % ADJ_GROUP == 0 || i + 1 == nbins) {   // FP32 groups of bins, as the kernel forms them
+                adj_add(acc, af);
+                adj_zero_f(af);
+            }
         }
         adj_finish(B, r + R_V20, r[R_SIGMA], acc, gv + 9 * b, ga + 8 * b);
     }

@@ -111,3 +111,23 @@ def test_model_forward_matches_oracle():
         err = np.abs(got - want).max(axis=(0, 1)) / np.abs(want).max()
         fwd = _forward_mask(th, ph, [bs])
         assert err[~fwd].max() <= 5e-9, err[~fwd].max()
+
+
+def test_aperture_vertices_gather_and_gradient():
+    """The index_select gather of aperture vertices equals advanced indexing, values and
+    gradient (torch on the CPU)."""
+    import torch
+    from abt_amd import autograd as AG
+    rng = np.random.default_rng(3)
+    V = torch.tensor(rng.normal(size=(12, 3)), requires_grad=True)
+    faces = torch.tensor(rng.integers(0, 12, size=(7, 3)))
+    fi = torch.tensor(rng.integers(1, 8, size=40))
+    R = torch.tensor(rng.normal(size=(4, 3, 3)))
+    o = torch.tensor(rng.integers(0, 4, size=40))
+    a = AG.aperture_vertices(V, faces, fi, R, o)
+    b = torch.einsum("bij,bvj->bvi", R[o], V[faces[fi - 1]])
+    np.testing.assert_allclose(a.detach().numpy(), b.detach().numpy(), rtol=1e-14, atol=1e-14)
+    G = torch.tensor(rng.normal(size=tuple(a.shape)))
+    ga, = torch.autograd.grad((a * G).sum(), V)
+    gb, = torch.autograd.grad((b * G).sum(), V)
+    np.testing.assert_allclose(ga.numpy(), gb.numpy(), rtol=1e-12, atol=1e-12)
