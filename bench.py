@@ -52,6 +52,8 @@ def main():
                     help="run the FP64 plain and FP64 batched comparisons of the orientation average (0: skip)")
     ap.add_argument("--adjoint-orient", type=int, default=64,
                     help="orientations of the adjoint workload (0: skip)")
+    ap.add_argument("--batch-tiles", default=None,
+                    help="fixed tile shape of the orientation-batched kernel, e.g. 16x1 (default: autotune)")
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -160,7 +162,8 @@ def main():
     del recs, amp, amp_ext, mueller, work, raw
     torch.cuda.empty_cache()
     if args.orient > 0:
-        out["orientation_average"] = orientation_average(args.orient, args.steps, lam, dev, bool(args.orient_fp64))
+        out["orientation_average"] = orientation_average(args.orient, args.steps, lam, dev, bool(args.orient_fp64),
+                                                         args.batch_tiles)
     if args.adjoint_orient > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_orient, lam, dev)
     line = json.dumps(out)
@@ -220,7 +223,7 @@ def _prepare_trees(w, lam, dev):
     return rb, re, nb, ne
 
 
-def orientation_average(norient, steps, lam, dev, with_fp64=True):
+def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=None):
     """Batched orientation averaging: `norient` orientations (uniform on SO(3), seeded) of
     a Gaussian-rough hexagonal column (radius 10 um, length 20 um, sigma 0.5 um, 2112
     facets), 10 internal reflections, 721 theta x 180 phi bins.  The whole job (every
@@ -254,9 +257,14 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True):
         return (torch.empty(16 * nbins, **f64), torch.empty(6 * norient, **f64),
                 torch.empty(16 * len(th), **f64))
 
-    # mixed precision: autotune, capture, replay
+    # mixed precision: autotune (or a fixed tile shape), capture, replay
     plan = lib.BatchPlan(g, boff, eoff, "mixed")
-    tune = plan.autotune(rb, re, sample=32)
+    if fixed_tiles:
+        tb, bpt = (int(v) for v in fixed_tiles.lower().split("x"))
+        plan.set_tiles(tb, bpt)
+        tune = {}
+    else:
+        tune = plan.autotune(rb, re, sample=32)
     tiles = plan.tiles()
     m, I, m1 = buffers()
     s = torch.cuda.Stream()
