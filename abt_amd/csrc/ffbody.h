@@ -425,7 +425,10 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     const bool fwd = !(fabs(uz) < FWD_DOT);
     const float ik = g.inv_krev_f;
     const float fk1 = (float)k1r * ik, fk2 = (float)k2r * ik;
-    const float fk3 = fmaf(rf[F_RR + 6], (float)ux, fmaf(rf[F_RR + 7], (float)uy, rf[F_RR + 8] * (float)uz));
+    // k_z in FP64: near k = +-e_y, 1 - k_y^2 = k_x^2 + k_z^2 is small and needs k_x, k_z to
+    // relative (not absolute) FP32 precision
+    const double* RR = r + R_RR;
+    const float fk3 = (float)fma(RR[6], ux, fma(RR[7], uy, RR[8] * uz));
     const float s2 = fmaf(fk1, fk1, fk3 * fk3);                   // 1 - k_y^2 (unclamped)
     float gg, hh;
     if (!kclamp && !fwd) {
