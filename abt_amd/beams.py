@@ -13,12 +13,18 @@ through `fi` (diff_mod.f90:529-531):
     verts     (nv, 3)  float64  facet vertices, in the facet's own order
 
 The same layout crosses the C-ABI (include/abt_diff.h, struct abt_beams).
-A file form (`save`/`load`, numpy .npz, no pickle) keeps beam sets
-reproducible across hosts.
+Two file forms keep beam sets reproducible across hosts: `save`/`load` (numpy .npz,
+no pickle) and `to_json`/`from_json`, a text form whose records carry the
+`outbeamtype` field names (ampl, vk7, prop_out, fov, verts) with the amplitude
+matrix written row by row as [[[re, im], [re, im]], [[re, im], [re, im]]], the
+order of the reference's own beam export (beam_loop_mod.f90:254-263).  Floats
+are written with Python's shortest round-trip repr, so a JSON round trip is exact.
 """
 from __future__ import annotations
 
 import dataclasses
+import json
+
 import numpy as np
 
 
@@ -87,6 +93,38 @@ class BeamSet:
     def load(path) -> "BeamSet":
         with np.load(path, allow_pickle=False) as z:
             return BeamSet(z["ampl"], z["vk7"], z["prop"], z["fov"], z["vert_off"], z["verts"])
+
+    def to_json(self, path):
+        a = self.ampl
+        beams = []
+        for b in range(self.n):
+            v = self.verts[self.vert_off[b]:self.vert_off[b + 1]]
+            # ampl(r, c) at pair index r + 2c of the Fortran column-major order
+            rows = [[[float(a[b, 2 * (r + 2 * c)]), float(a[b, 2 * (r + 2 * c) + 1])] for c in range(2)]
+                    for r in range(2)]
+            beams.append({"ampl": rows, "vk7": [float(x) for x in self.vk7[b]],
+                          "prop_out": [float(x) for x in self.prop[b]], "fov": float(self.fov[b]),
+                          "verts": [[float(x) for x in p] for p in v]})
+        with open(path, "w") as f:
+            json.dump({"outbeams": beams}, f)
+
+    @staticmethod
+    def from_json(path) -> "BeamSet":
+        with open(path) as f:
+            beams = json.load(f)["outbeams"]
+        n = len(beams)
+        ampl = np.zeros((n, 8))
+        for b, rec in enumerate(beams):
+            for r in range(2):
+                for c in range(2):
+                    ampl[b, 2 * (r + 2 * c):2 * (r + 2 * c) + 2] = rec["ampl"][r][c]
+        nv = [len(rec["verts"]) for rec in beams]
+        off = np.zeros(n + 1, np.int64)
+        np.cumsum(nv, out=off[1:])
+        verts = np.array([p for rec in beams for p in rec["verts"]], dtype=np.float64).reshape(-1, 3)
+        return BeamSet(ampl, [rec["vk7"] for rec in beams] if n else np.zeros((0, 3)),
+                       [rec["prop_out"] for rec in beams] if n else np.zeros((0, 3)),
+                       [rec["fov"] for rec in beams], off, verts)
 
 
 def concat(sets) -> BeamSet:

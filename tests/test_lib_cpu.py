@@ -103,3 +103,28 @@ def test_bad_arguments_raise():
     bs = random_triangles(2, seed=1)
     with pytest.raises(lib.AbtError, match="wavelength"):
         lib.prepare_beams_host(bs, -1.0)
+
+
+def test_beam_set_files_round_trip(tmp_path):
+    """npz and JSON forms of a beam set (triangles and a quadrilateral) round-trip exactly;
+    the JSON amplitude matrix is written row by row: [[a11, a12], [a21, a22]]."""
+    from abt_amd.beams import BeamSet, concat
+    tri = random_triangles(5, seed=3, box=10.0)
+    quad = BeamSet(np.arange(8.0)[None] + 0.25, [[0.0, 1.0, 0.0]], [[0.0, 0.0, -1.0]], [0.7], [0, 4],
+                   [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0.0]])
+    bs = concat([tri, quad])
+    for save, load, name in ((BeamSet.save, BeamSet.load, "b.npz"), (BeamSet.to_json, BeamSet.from_json, "b.json")):
+        save(bs, tmp_path / name)
+        got = load(tmp_path / name)
+        for f in ("ampl", "vk7", "prop", "fov", "vert_off", "verts"):
+            np.testing.assert_array_equal(getattr(got, f), getattr(bs, f))
+    import json
+    rec = json.load(open(tmp_path / "b.json"))["outbeams"][-1]
+    a = bs.ampl_complex()[-1]
+    for r in range(2):
+        for c in range(2):
+            assert rec["ampl"][r][c] == [a[r, c].real, a[r, c].imag]
+    assert rec["prop_out"] == [0.0, 0.0, -1.0] and len(rec["verts"]) == 4
+    empty = tmp_path / "e.json"
+    BeamSet.empty().to_json(empty)
+    assert BeamSet.from_json(empty).n == 0
