@@ -45,9 +45,13 @@
  * and the triangle closed form), ampl_to_mueller (Jones-to-Mueller Kronecker
  * form), simpne (exact on quadratics), diffraction (Babinet cross-section,
  * forward-direction non-depolarisation, shift theorem).  The per-beam sign
- * flip (diff_mod.f90:561-571) and the theta_i forward fallback
- * (diff_mod.f90:630-635) have no independent pin: parity unpinned for those
- * two conventions beyond code reading.
+ * flip (diff_mod.f90:561-571) is pinned by its consequence that reversing an
+ * aperture's vertex order negates the diffracted amplitude, and the theta_i
+ * forward fallback (diff_mod.f90:630-635) by the smoothness of the far field
+ * across the edge of the fallback cone; each check also fails on an oracle
+ * rebuilt with the corresponding mistake planted.  The absolute sign of the
+ * flip (which of the two orientations is negated) is not observable in any
+ * Mueller matrix and stays pinned by code reading only.
  */
 #define _GNU_SOURCE
 #include <math.h>

@@ -280,3 +280,94 @@ def test_diff_main_is_ordered_sum_and_thread_independent(orc):
     for b in range(bs.n):
         s = s + orc.diffraction(theta, phi, x, y, z, LAM, bs, b)
     np.testing.assert_array_equal(orc.complex_to_raw(s), b1)
+
+
+def _planted(tmp_path, old, new):
+    """The oracle rebuilt from its source with one planted mistake, to show that a pin
+    catches it (compiled like oracle/Makefile, into tmp_path)."""
+    import ctypes
+    import os
+    import subprocess
+    import oracle as O
+    src_path = os.path.join(os.path.dirname(O.__file__), "diff_oracle.c")
+    src = open(src_path).read()
+    assert old in src
+    p = tmp_path / "planted.c"
+    p.write_text(src.replace(old, new))
+    out = tmp_path / "libplanted.so"
+    subprocess.run(["gcc", "-O2", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-fopenmp", "-shared",
+                    str(p), "-o", str(out), "-lm"], check=True)
+    L = ctypes.CDLL(str(out))
+    O._setup(L)
+    return L
+
+
+def _reversal_negates(orc):
+    """Max over beams of |A(reversed vertex order) + A| / max|A| and whether the
+    anti-parallel flag toggled for every beam."""
+    from abt_amd.workloads import grid, random_triangles
+    th, ph = grid((0, 10, 180), (0, 20, 340))
+    x, y, z = orc.make_far_field_bins(th, ph)
+    bs = random_triangles(12, seed=4, box=5.0)
+    worst, toggles = 0.0, True
+    for b in range(bs.n):
+        v = bs.verts[bs.vert_off[b]:bs.vert_off[b + 1]]
+        rv = BeamSet(bs.ampl[b:b + 1], bs.vk7[b:b + 1], bs.prop[b:b + 1], bs.fov[b:b + 1], [0, 3], v[::-1].copy())
+        a0 = orc.diffraction(th, ph, x, y, z, LAM, bs, b)
+        a1 = orc.diffraction(th, ph, x, y, z, LAM, rv, 0)
+        f0 = orc.beam_frame(bs.ampl[b], bs.vk7[b], bs.prop[b], bs.fov[b], v)["anti_parallel"]
+        f1 = orc.beam_frame(bs.ampl[b], bs.vk7[b], bs.prop[b], bs.fov[b], v[::-1].copy())["anti_parallel"]
+        toggles &= f0 != f1
+        worst = max(worst, np.abs(a0 + a1).max() / np.abs(a0).max())
+    return worst, toggles
+
+
+def test_vertex_order_reversal_negates_amplitude(orc, tmp_path, monkeypatch):
+    """Reversing an aperture's vertex order flips the aperture frame (get_rotation_matrix2,
+    diff_mod.f90:274-287) and toggles the anti-parallel sign flip of the amplitude matrix
+    (diff_mod.f90:561-571); together they negate the diffracted amplitude at every bin.
+    Without the sign flip the amplitude would be unchanged instead: planted in the oracle,
+    that mistake fails this check."""
+    worst, toggles = _reversal_negates(orc)
+    assert toggles and worst <= 1e-8, worst
+    L = _planted(tmp_path, "for (int q = 0; q < 8; q++) bf->ampl[q] = bf->anti_parallel ? -ampl[q] : ampl[q];",
+                 "for (int q = 0; q < 8; q++) bf->ampl[q] = ampl[q];")
+    import oracle as O
+    monkeypatch.setattr(O, "_lib", L)
+    worst, _ = _reversal_negates(O)
+    assert worst > 1.0, worst
+
+
+def _fallback_edge_jump(orc):
+    """Largest second difference along theta at the edge of the forward-fallback cone
+    (|cos theta| = 0.999, theta = 2.5625 deg), over that elsewhere in the band."""
+    from abt_amd.workloads import random_triangles
+    th_deg = np.r_[0.0, 0.5, 1.0, 1.5, 2.0, np.arange(2.40, 2.73, 0.02)]
+    th = th_deg * D2R
+    ph = np.array([0.0, 37.0, 123.0, 250.0]) * D2R
+    x, y, z = orc.make_far_field_bins(th, ph)
+    bs = random_triangles(8, seed=8, box=20.0)
+    bs.prop[:] = _unit(np.array([0.05, -0.03, -1.0]))
+    band = th_deg >= 2.39
+    ib = np.searchsorted(th_deg[band], np.degrees(np.arccos(0.9990000128746033)))
+    ratio = 0.0
+    for b in range(bs.n):
+        a = orc.diffraction(th, ph, x, y, z, LAM, bs, b)[:, :, band, :]
+        d2 = np.diff(a, n=2, axis=2)
+        edge = np.abs(d2[:, :, ib - 2:ib + 1]).max()
+        other = np.abs(np.delete(d2, [ib - 2, ib - 1, ib], axis=2)).max()
+        ratio = max(ratio, edge / other)
+    return ratio
+
+
+def test_forward_fallback_is_continuous(orc, tmp_path, monkeypatch):
+    """Inside 2.56 deg of the incidence axis the scattering-plane vector hc comes from the bin
+    (same phi, theta_i) (diff_mod.f90:630-635).  That bin lies in the same phi plane, so hc
+    barely changes across the cone's edge and the far field stays smooth there: the second
+    differences along theta that touch the edge are no larger than elsewhere.  A fallback to
+    the wrong bin (phi index 0), planted in the oracle, breaks the smoothness."""
+    assert _fallback_edge_jump(orc) <= 1.2
+    L = _planted(tmp_path, "int64_t bt = i + nphi * theta_i;", "int64_t bt = nphi * theta_i;")
+    import oracle as O
+    monkeypatch.setattr(O, "_lib", L)
+    assert _fallback_edge_jump(O) > 3.0
