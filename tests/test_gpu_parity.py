@@ -131,6 +131,9 @@ def test_fp64_quadrilateral_apertures(orc):
     rb, _ = orc.diff_main(th, ph, LAM, b, BeamSet.empty(), nthreads=8)
     gb, _ = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision="fp64")
     _check_fp64(gb, rb, _forward_mask(th, ph, [b]))
+    # the mixed kernel's polygon path (vertex exponentials from the vertex records)
+    mb, _ = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision="mixed")
+    check_mixed(mb, rb, _forward_mask(th, ph, [b]))
 
 
 def test_empty_trees_give_zeros():
