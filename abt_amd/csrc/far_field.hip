@@ -48,7 +48,12 @@ __global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __
                                                      double* __restrict__ out,
                                                      double* __restrict__ work)
 {
-    __shared__ double tile[TB * REC];
+    // Mixed kernel: two record tiles; the next tile's records are loaded into registers
+    // while the current one is summed, then stored to the other buffer (one barrier per
+    // tile).  The FP64 bodies need their registers: one tile, loaded between barriers.
+    constexpr bool DB = PREC == 1;
+    __shared__ double tile[(DB ? 2 : 1) * TB * REC];
+    constexpr int PF = DB ? (TB * REC + BLOCK - 1) / BLOCK : 1;   // doubles per lane per tile
     const int tid = threadIdx.x;
     const int bin = blockIdx.x * BLOCK + tid;
     const bool active = bin < g.nbins;
@@ -64,12 +69,36 @@ __global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __
     for (int q = 0; q < 8; q++) acc[q] = 0.0;
     const int64_t b0 = (int64_t)blockIdx.y * per_chunk;
     const int64_t b1 = min(nb, b0 + per_chunk);
+    if (DB) {
+        // the bin's inputs must have arrived before the tile loop: a use inside it would make
+        // the compiler wait there for the older loads, and with them for the prefetched tile
+        asm volatile("" : "+v"(bi.x), "+v"(bi.y), "+v"(bi.z), "+v"(bi.xt), "+v"(bi.yt), "+v"(bi.zt));
+    }
+    if (DB && b0 < b1) {
+        const int nt0 = (int)min((int64_t)TB, b1 - b0);
+        const double* src = recs + b0 * REC;
+        for (int q = tid; q < nt0 * REC; q += BLOCK) tile[q] = src[q];
+    }
+    if (DB) __syncthreads();
+    int buf = 0;
     for (int64_t t0 = b0; t0 < b1; t0 += TB) {
         const int nt = (int)min((int64_t)TB, b1 - t0);
-        __syncthreads();
-        const double* src = recs + t0 * REC;
-        for (int q = tid; q < nt * REC; q += BLOCK) tile[q] = src[q];
-        __syncthreads();
+        const int64_t tn = t0 + TB;
+        const int ntn = DB && tn < b1 ? (int)min((int64_t)TB, b1 - tn) : 0;
+        const double* srcn = recs + (ntn ? tn : 0) * REC;
+        double pre[PF];
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int q = tid + k * BLOCK;
+            pre[k] = q < ntn * REC ? srcn[q] : 0.0;
+        }
+        if (!DB) {
+            __syncthreads();
+            const double* src = recs + t0 * REC;
+            for (int q = tid; q < nt * REC; q += BLOCK) tile[q] = src[q];
+            __syncthreads();
+        }
+        const double* cur = tile + buf * (TB * REC);
         if (active) {
             if (PREC == 1) {
                 // FP32 partial sums over groups of MIXED_GROUP beams, added into FP64
@@ -78,14 +107,24 @@ __global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __
 #pragma unroll
                     for (int q = 0; q < 8; q++) accf[q] = 0.f;
                     const int g1 = min(nt, g0 + MIXED_GROUP);
-                    for (int bb = g0; bb < g1; bb++) beam_bin_mixed(tile + bb * REC, vrecs, g, bi, fast_diff, accf);
+                    for (int bb = g0; bb < g1; bb++) beam_bin_mixed(cur + bb * REC, vrecs, g, bi, fast_diff, accf);
 #pragma unroll
                     for (int q = 0; q < 8; q++) acc[q] += (double)accf[q];
                 }
             } else {
                 for (int bb = 0; bb < nt; bb++)
-                    beam_bin_fp64(tile + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, PREC == 2);
+                    beam_bin_fp64(cur + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, PREC == 2);
             }
+        }
+        if (DB) {
+            double* nxt = tile + (buf ^ 1) * (TB * REC);
+#pragma unroll
+            for (int k = 0; k < PF; k++) {
+                const int q = tid + k * BLOCK;
+                if (q < ntn * REC) nxt[q] = pre[k];
+            }
+            __syncthreads();
+            buf ^= 1;
         }
     }
     if (!active) return;
