@@ -11,6 +11,9 @@
 int abt_fail(int code, const char* fmt, ...);
 // Select the library's device for the calling thread; *cus = its CU count.
 int abt_ensure_device(int* cus);
+// The calling thread's stream and its grow-only device buffer `slot` (at least `bytes`
+// long; slot 8 belongs to the visibility entry points).
+int abt_thread_staging(int slot, size_t bytes, hipStream_t* st, void** p);
 
 #define HIPCHK(expr)                                                                      \
     do {                                                                                  \

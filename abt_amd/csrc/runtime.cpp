@@ -71,7 +71,7 @@ struct DevBuf {
 // per-thread staging: the host entry points may run concurrently on several threads
 struct ThreadState {
     hipStream_t st = nullptr;
-    DevBuf bufs[8];
+    DevBuf bufs[9];   // [8]: the visibility entry points
     std::vector<double> hrec;
 };
 
@@ -177,6 +177,20 @@ int abt_ensure_device(int* cus)
     int rc = ensure_device();
     if (!rc && cus) *cus = g_ctx.cus;
     return rc;
+}
+
+int abt_thread_staging(int slot, size_t bytes, hipStream_t* st, void** p)
+{
+    if (slot < 0 || slot >= 9) return fail(ABT_ERR_ARG, "staging slot %d out of range", slot);
+    int rc = ensure_device();
+    if (rc) return rc;
+    ThreadState& ts = tstate();
+    rc = stream_of(ts);
+    if (!rc) rc = buf_ensure(ts.bufs[slot], bytes);
+    if (rc) return rc;
+    *st = ts.st;
+    *p = ts.bufs[slot].p;
+    return 0;
 }
 
 // ---------------------------------------------------------------------------

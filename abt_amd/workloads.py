@@ -400,3 +400,51 @@ def orientation_workload(norient, particle="rough_column", seed=0, la=0.532, dev
     t = trace_orientations(verts, faces, rots, la=la, m=m, rec=rec, device=device)
     t["verts0"], t["faces"], t["rots"] = verts, faces, rots
     return t
+
+
+# ---- facet-visibility workloads (the near-field tracer's find_vis_* inputs) ----
+
+def hex_column_apertures(radius=5.0, length=10.0, nfhr=6, nfpl=12, sigma=0.0, seed=0):
+    """hex_column plus its apertures: the 6 prism faces (ids 1..6), the basal faces
+    z = -length/2 (7) and z = +length/2 (8).  Returns (verts, faces, ap, ap_normals)."""
+    verts, faces = hex_column(radius, length, nfhr, nfpl, sigma, seed=seed)
+    ang = np.arange(6) * np.pi / 3 + np.pi / 6
+    side = 2 * nfhr * nfpl
+    base = 6 * nfhr * nfhr
+    ap = np.concatenate([np.repeat(np.arange(1, 7), side), np.full(base, 7), np.full(base, 8)])
+    assert len(ap) == len(faces)
+    normals = np.concatenate([np.stack([np.cos(ang), np.sin(ang), np.zeros(6)], 1),
+                              [[0.0, 0.0, -1.0], [0.0, 0.0, 1.0]]])
+    return verts, faces, ap, normals
+
+
+def vis_workload(nbeam, mode="int", particle="rough_column", seed=0):
+    """`nbeam` beams for find_vis_*: the particle in a random orientation per beam (uniform on
+    SO(3), seeded) seen along -z, and for internal (external) beams the facets of one
+    up-facing (down-facing) aperture, picked at random, as the illuminating facets.
+    Returns (topology VisGeometry, v (nbeam,3,nv), mid (nbeam,nf,3), nz (nbeam,nf),
+    ap_nz (nbeam,na), in_off (nbeam+1), field_in (1-based), beam_mid_z (nbeam))."""
+    from .visibility import VisGeometry
+    radius, length, nfhr, nfpl, sigma = PARTICLES[particle][:5]
+    verts, faces, ap, apn = hex_column_apertures(radius, length, nfhr, nfpl, sigma, seed=seed)
+    topo = VisGeometry.from_mesh(verts, faces, ap, apn)
+    rots = random_rotations(nbeam, seed + 7)
+    rng = np.random.default_rng(seed + 11)
+    nf, nv, na = topo.nf, topo.nv, topo.na
+    v = np.empty((nbeam, 3, nv)); mid = np.empty((nbeam, nf, 3))
+    nz = np.empty((nbeam, nf)); ap_nz = np.empty((nbeam, na))
+    in_off = np.zeros(nbeam + 1, np.int64); fins = []; bmz = np.empty(nbeam)
+    for b in range(nbeam):
+        g = topo.rotated(rots[b])
+        v[b], mid[b], nz[b], ap_nz[b] = g.v.T, g.mid, g.nz, g.ap_nz
+        bmz[b] = g.v[:, 2].max() + 1.0
+        if mode == "inc":
+            fin = np.zeros(0, np.int64)
+        else:
+            want = g.ap_nz > 0.01 if mode == "int" else g.ap_nz < -0.01
+            a = rng.choice(np.nonzero(want)[0]) + 1
+            fin = np.nonzero(topo.ap == a)[0] + 1
+        fins.append(fin)
+        in_off[b + 1] = in_off[b] + len(fin)
+    field_in = np.concatenate(fins) if fins else np.zeros(0, np.int64)
+    return topo, v, mid, nz, ap_nz, in_off, field_in, bmz

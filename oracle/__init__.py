@@ -193,3 +193,84 @@ def finalise(theta, phi, la, raw_beam, raw_ext, scaling=False, beam_energy_out=1
                 mueller_ext=m[2].reshape(16, nb), mueller_1d=m1[0].reshape(16, nth),
                 mueller_1d_beam=m1[1].reshape(16, nth), mueller_1d_ext=m1[2].reshape(16, nth),
                 ampl_beam=rb, ampl_ext=re, **dict(zip(keys, params)))
+
+
+# ---- facet visibility (vis_oracle.c) ----
+
+def _setup_vis(L):
+    vp = ctypes.c_void_p
+    L.oracle_vis_boxes.argtypes = [_i64, vp, vp, vp]
+    L.oracle_find_vis.argtypes = [ctypes.c_int, _i64, vp, _i64, vp, vp, vp, vp, vp, _i64, vp, _i64, vp,
+                                  ctypes.c_double, vp, vp, vp, vp, vp, vp]
+    L.oracle_find_vis_batch.argtypes = [ctypes.c_int, _i64, _i64, vp, _i64, vp, vp, vp, vp, vp, _i64, vp,
+                                        vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_int]
+
+
+_vis_ready = False
+
+
+def _vlib():
+    global _vis_ready
+    L = lib()
+    if not _vis_ready:
+        _setup_vis(L)
+        _vis_ready = True
+    return L
+
+
+def _a(x, dt):
+    return np.ascontiguousarray(x, dtype=dt)
+
+
+def vis_boxes(v):
+    """Box midpoints of beam_aligned_bounding_boxes + compute_geometry_midpoints: (bx[16], by[16])."""
+    vc = _a(np.asarray(v).T, np.float64)
+    bx, by = np.empty(16), np.empty(16)
+    _vlib().oracle_vis_boxes(vc.shape[1], vc.ctypes.data, bx.ctypes.data, by.ctypes.data)
+    return bx, by
+
+
+def find_vis(mode, geom, field_in=None, beam_mid_z=0.0, with_counts=False):
+    """find_vis_int (mode 0) / find_vis_ext (1) / find_vis_inc (2) of one beam; geom has
+    v (nv,3), f_off, f_vi (1-based), mid (nf,3), nz, ap (1-based), ap_nz."""
+    mode = {"int": 0, "ext": 1, "inc": 2}.get(mode, mode)
+    vc = _a(np.asarray(geom.v).T, np.float64)
+    off, vi, mid = _a(geom.f_off, np.int64), _a(geom.f_vi, np.int64), _a(geom.mid, np.float64)
+    nz, ap, apn = _a(geom.nz, np.float64), _a(geom.ap, np.int64), _a(geom.ap_nz, np.float64)
+    fin = _a(field_in if field_in is not None else [0], np.int64)
+    nfin = 0 if field_in is None else len(field_in)
+    nf = len(off) - 1
+    inb, vis, shad = np.zeros(nf, np.uint8), np.zeros(nf, np.uint8), np.zeros(nf, np.uint8)
+    dist, idb, cnt = np.zeros(nf), np.zeros(nf, np.int64), np.zeros(2, np.int64)
+    _vlib().oracle_find_vis(mode, vc.shape[1], vc.ctypes.data, nf, off.ctypes.data, vi.ctypes.data,
+                            mid.ctypes.data, nz.ctypes.data, ap.ctypes.data, len(apn), apn.ctypes.data,
+                            nfin, fin.ctypes.data, float(beam_mid_z), inb.ctypes.data, vis.ctypes.data,
+                            dist.ctypes.data, idb.ctypes.data, shad.ctypes.data, cnt.ctypes.data)
+    out = dict(in_beam=inb.astype(bool), is_vis=vis.astype(bool), dist_beam=dist, id_beam=idb,
+               is_shad=shad.astype(bool))
+    if with_counts:
+        out["counts"] = cnt
+    return out
+
+
+def find_vis_batch(mode, topo, v, mid, nz, ap_nz, in_off, field_in, beam_mid_z=None, nthreads=0):
+    """Many beams over one topology (OpenMP over beams): v (nbeam,3,nv), mid (nbeam,nf,3),
+    nz (nbeam,nf), ap_nz (nbeam,na), in_off (nbeam+1), field_in (1-based).  Returns the
+    outputs as (nbeam, nf) arrays and the per-beam counts (nbeam, 2)."""
+    mode = {"int": 0, "ext": 1, "inc": 2}.get(mode, mode)
+    v, mid, nz, apn = _a(v, np.float64), _a(mid, np.float64), _a(nz, np.float64), _a(ap_nz, np.float64)
+    off, vi, ap = _a(topo.f_off, np.int64), _a(topo.f_vi, np.int64), _a(topo.ap, np.int64)
+    ino, fin = _a(in_off, np.int64), _a(field_in if len(field_in) else [0], np.int64)
+    nbeam, nvv = v.shape[0], v.shape[2]
+    nf = len(off) - 1
+    bmz = None if beam_mid_z is None else _a(beam_mid_z, np.float64)
+    inb, vis, shad = (np.zeros((nbeam, nf), np.uint8) for _ in range(3))
+    dist, idb, cnt = np.zeros((nbeam, nf)), np.zeros((nbeam, nf), np.int64), np.zeros((nbeam, 2), np.int64)
+    _vlib().oracle_find_vis_batch(mode, nbeam, nvv, v.ctypes.data, nf, off.ctypes.data, vi.ctypes.data,
+                                  mid.ctypes.data, nz.ctypes.data, ap.ctypes.data, apn.shape[1],
+                                  apn.ctypes.data, ino.ctypes.data, fin.ctypes.data,
+                                  None if bmz is None else bmz.ctypes.data, inb.ctypes.data,
+                                  vis.ctypes.data, dist.ctypes.data, idb.ctypes.data, shad.ctypes.data,
+                                  cnt.ctypes.data, int(nthreads))
+    return dict(in_beam=inb.astype(bool), is_vis=vis.astype(bool), dist_beam=dist, id_beam=idb,
+                is_shad=shad.astype(bool)), cnt

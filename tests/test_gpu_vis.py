@@ -1,0 +1,161 @@
+"""Facet visibility on the GPU (abt_find_vis, abt_find_vis_dev) against the oracle's
+restatement of find_vis_int / find_vis_ext / find_vis_inc (beam_loop_mod.f90:2037-2609).
+
+The bar is bit-identical output: in_beam, is_vis, is_shad, id_beam and dist_beam for every
+facet, including the values the reference leaves behind on blocked facets, and the work
+counters equal to the oracle's.  Cases: the rough column (2112 facets) in random beam frames
+for all three modes, the layered two-body scenes of the oracle pins, polygon facets
+(quadrilaterals and hexagons), a repeated illuminating facet, degenerate boxes (every vertex
+on one x), and a full 2048-beam batch.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from abt_amd import visibility as V
+from abt_amd.workloads import hex_column_apertures, random_rotations, vis_workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(r, ref, where=""):
+    for k in ("in_beam", "is_vis", "is_shad", "id_beam", "dist_beam"):
+        a, b = np.asarray(r[k]), np.asarray(ref[k])
+        bad = np.nonzero(a != b)
+        assert a.shape == b.shape and bad[0].size == 0, f"{where} {k}: {bad[0].size} differ, first {bad[0][:5]}"
+
+
+def _geom(topo, v, mid, nz, apn, b):
+    g = V.VisGeometry(v[b].T.copy(), topo.f_off, topo.f_vi, mid[b], nz[b], topo.ap, apn[b])
+    return g
+
+
+@pytest.mark.parametrize("mode", ["int", "ext", "inc"])
+def test_single_beam_host_api_matches_oracle(mode):
+    topo, v, mid, nz, apn, ino, fin, bmz = vis_workload(6, mode, seed=3)
+    for b in range(6):
+        g = _geom(topo, v, mid, nz, apn, b)
+        f = fin[ino[b]:ino[b + 1]]
+        print(f"vis host {mode} beam {b}", flush=True)
+        r = V.find_vis(mode, g, f if mode != "inc" else None, bmz[b])
+        ref = O.find_vis(mode, g, f if mode != "inc" else None, bmz[b])
+        _same(r, ref, f"{mode} beam {b}")
+
+
+def _batch(mode, topo, v, mid, nz, apn, ino, fin, bmz, counts=True):
+    import torch
+    dev = torch.device("cuda:0")
+    t = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    B = V.VisBatch(mode, topo, t(v), t(mid), t(nz), t(apn), t(ino, torch.int64),
+                   t(fin if len(fin) else np.zeros(1, np.int64), torch.int64), t(bmz))
+    B.run(counts=counts)
+    torch.cuda.synchronize()
+    r = {k: x.cpu().numpy() for k, x in B.results().items()}
+    return r, B.counts.cpu().numpy()
+
+
+@pytest.mark.parametrize("mode", ["int", "ext", "inc"])
+def test_batched_device_api_matches_oracle(mode):
+    args = vis_workload(96, mode, seed=5)
+    print(f"vis batch {mode} 96 beams", flush=True)
+    r, cnt = _batch(mode, *args)
+    ref, rc = O.find_vis_batch(mode, *args)
+    _same(r, ref, mode)
+    np.testing.assert_array_equal(cnt, rc.sum(0))
+
+
+def _scene(bodies, R):
+    verts, faces, ap, apn = [], [], [], []
+    nv = na = 0
+    for radius, length, nfhr, nfpl, off in bodies:
+        v, f, a, n = hex_column_apertures(radius, length, nfhr, nfpl)
+        verts.append(v + np.asarray(off, float))
+        faces.append(f + nv)
+        ap.append(a + na)
+        apn.append(n)
+        nv += len(v)
+        na += len(n)
+    return V.VisGeometry.from_mesh(np.concatenate(verts), np.concatenate(faces), np.concatenate(ap),
+                                   np.concatenate(apn)).rotated(R)
+
+
+def test_layered_scenes_match_oracle():
+    R = random_rotations(1, 77)[0]
+    small = np.array([[1, 0, 0], [0, 0.9986, -0.0523], [0, 0.0523, 0.9986]])
+    scenes = [
+        ("inc", _scene([(6.0, 4.0, 5, 3, (0, 0, 0)), (2.5, 2.0, 3, 2, (3.5, 1.0, 5.0))], small), None),
+        ("int", _scene([(2.0, 1.0, 2, 1, (2.0, 1.5, -1.0)), (6.0, 8.0, 4, 3, (0, 0, 0))], small), (8, 16)),
+        ("ext", _scene([(6.0, 3.0, 4, 2, (0, 0, 0)), (3.0, 2.0, 3, 1, (2.0, 0.5, 6.0))], small), (15,)),
+        ("int", _scene([(5.0, 6.0, 3, 3, (0, 0, 0)), (1.5, 1.0, 2, 1, (1.0, 0.0, 0.0))], R), (8, 16)),
+    ]
+    for i, (mode, g, aps) in enumerate(scenes):
+        f = None if aps is None else np.nonzero(np.isin(g.ap, aps))[0] + 1
+        bz = g.v[:, 2].max() + 1.0
+        print(f"vis scene {i} {mode}", flush=True)
+        _same(V.find_vis(mode, g, f, bz), O.find_vis(mode, g, f, bz), f"scene {i}")
+
+
+def _polygon_prism(R):
+    """A hexagonal prism with quadrilateral sides and hexagonal ends (nv = 4 and 6), next to
+    a triangulated column, so polygons block and are blocked."""
+    ang = np.arange(6) * np.pi / 3
+    ring = np.stack([3 * np.cos(ang), 3 * np.sin(ang)], 1)
+    v = np.concatenate([np.c_[ring, np.full(6, -2.0)], np.c_[ring, np.full(6, 2.0)]]) + [1.0, 0.5, 4.0]
+    faces = [[s, (s + 1) % 6, 6 + (s + 1) % 6, 6 + s] for s in range(6)]
+    faces += [list(range(5, -1, -1)), list(range(6, 12))]
+    nrm = np.concatenate([np.stack([np.cos(ang + np.pi / 6), np.sin(ang + np.pi / 6), np.zeros(6)], 1),
+                          [[0, 0, -1.0], [0, 0, 1.0]]])
+    v2, f2, a2, n2 = hex_column_apertures(6.0, 3.0, 3, 2)
+    verts = np.concatenate([v, v2])
+    allf = faces + [list(t + len(v)) for t in f2]
+    ap = np.concatenate([np.arange(1, 9), a2 + 8])
+    return V.VisGeometry.from_mesh(verts, allf, ap, np.concatenate([nrm, n2])).rotated(R)
+
+
+@pytest.mark.parametrize("mode", ["int", "ext", "inc"])
+def test_polygon_facets_match_oracle(mode):
+    for s in range(3):
+        R = random_rotations(1, 300 + s)[0] if s else np.eye(3)
+        g = _polygon_prism(R)
+        want = g.ap_nz > 0.01 if mode == "int" else g.ap_nz < -0.01
+        f = None
+        if mode != "inc":
+            a = np.nonzero(want)[0][0] + 1
+            f = np.nonzero(g.ap == a)[0] + 1
+        bz = g.v[:, 2].max() + 1.0
+        print(f"vis polygon {mode} {s}", flush=True)
+        _same(V.find_vis(mode, g, f, bz), O.find_vis(mode, g, f, bz), f"polygon {mode} {s}")
+
+
+def test_repeated_and_single_illuminating_facets():
+    topo, v, mid, nz, apn, ino, fin, bmz = vis_workload(2, "int", seed=9)
+    g = _geom(topo, v, mid, nz, apn, 0)
+    f = fin[ino[0]:ino[1]]
+    for fi in (np.r_[f, f[::-1], f[:3]], f[:1], np.arange(1, g.nf + 1)):
+        print(f"vis field_in of {len(fi)}", flush=True)
+        _same(V.find_vis("int", g, fi), O.find_vis("int", g, fi), f"field_in {len(fi)}")
+
+
+def test_degenerate_boxes_and_ties():
+    """Every vertex on one x (all boxes share x; midpoint distances tie across box rows) and
+    a flat sheet (every facet at one depth)."""
+    topo, v, mid, nz, apn, ino, fin, bmz = vis_workload(1, "int", seed=2)
+    g = _geom(topo, v, mid, nz, apn, 0)
+    flat = V.VisGeometry(g.v * [0.0, 1.0, 1.0], g.f_off, g.f_vi, g.mid * [0.0, 1.0, 1.0], g.nz, g.ap, g.ap_nz)
+    sheet = V.VisGeometry(g.v * [1.0, 1.0, 0.0], g.f_off, g.f_vi, g.mid * [1.0, 1.0, 0.0], g.nz, g.ap, g.ap_nz)
+    f = fin[ino[0]:ino[1]]
+    for name, h in (("flat-x", flat), ("sheet", sheet)):
+        for mode in ("int", "ext", "inc"):
+            print(f"vis degenerate {name} {mode}", flush=True)
+            ff = None if mode == "inc" else f
+            _same(V.find_vis(mode, h, ff, 3.0), O.find_vis(mode, h, ff, 3.0), f"{name} {mode}")
+
+
+def test_full_batch_matches_oracle():
+    """A full benchmark-size batch: 2048 internal beams of the rough column."""
+    args = vis_workload(2048, "int", seed=1)
+    print("vis batch int 2048 beams", flush=True)
+    r, cnt = _batch("int", *args)
+    ref, rc = O.find_vis_batch("int", *args, nthreads=16)
+    _same(r, ref, "2048 beams")
+    np.testing.assert_array_equal(cnt, rc.sum(0))

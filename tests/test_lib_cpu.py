@@ -16,7 +16,8 @@ LAM = 0.532
 
 
 def _declared():
-    src = open(os.path.join(ROOT, "include", "abt_diff.h")).read()
+    src = "".join(open(os.path.join(ROOT, "include", h)).read()
+                  for h in sorted(os.listdir(os.path.join(ROOT, "include"))) if h.endswith(".h"))
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(abt_[a-z0-9_]+)\s*\(", src)))
 
