@@ -15,18 +15,16 @@
 //   k_vis_boxes   one workgroup per beam: x-y extents (block min/max), box
 //                 midpoints, the illuminating facets' extent;
 //   k_vis_facets  one lane per facet: box, candidate mask, cull and shadow
-//                 flags, initial outputs, and the facet's blocker record
-//                 (mid_z, aperture, vertex count, then per edge the start
-//                 vertex and the edge normal the reference forms per pair --
-//                 the same two subtractions, so the same bits);
+//                 flags, initial outputs;
 //   k_vis_map     field_in -> mapping (last position wins, as the reference's
 //                 sequential store);
 //   k_vis_lists   one workgroup per beam: per box, the ascending candidate
 //                 list and the ascending list of tested facets (block-wide
 //                 ballot scans, so list order is facet order);
 //   k_vis_main    one wave per 64 tested facets of one box: every lane of the
-//                 wave scans the same candidate list in facet order, so the
-//                 candidate's record is read at a wave-uniform address; a lane
+//                 wave scans the same candidate list in facet order; tiles of
+//                 candidates (depth, aperture, projected vertices) are gathered
+//                 into LDS by parallel loads and read back as broadcasts; a lane
 //                 stops at its blocker, the wave when all lanes have stopped.
 // All arithmetic is the reference's FP64 operations with contraction off and
 // comparisons in the same sense; min/max reductions are exact, so results do
@@ -47,6 +45,9 @@ constexpr int NB = 16;                                // 4 x 4 boxes
 constexpr int BLK = 256;
 constexpr int WAVE = 64;
 constexpr int OFFS = 2 * (NB + 1) + 2;                // per-beam list offsets (padded)
+constexpr int TC = 16;                                // candidates per LDS tile of the scan
+constexpr int NVS = 8;                                // vertices per candidate staged in LDS
+constexpr int SLOT = 4 + 2 * NVS;                     // doubles per staged candidate
 constexpr int64_t BEAM_CHUNK = 65535;                 // beams per launch (grid.y)
 constexpr double FAC = 1.100000023841858;             // fac = 1.1, a REAL*4 literal (:2679)
 constexpr double C001 = 0.009999999776482582;         // 0.01, a REAL*4 literal
@@ -63,10 +64,8 @@ struct Work {
     int32_t* fmask;     // [nbeam][nf] bit k: candidate blocker in box k
     int32_t* mapping;   // [nbeam][nf] 1-based position in field_in (the last), 0: not a beam facet
     int32_t* offs;      // [nbeam][OFFS] candidate-list offsets [0..NB], tested-list offsets [NB+1..2NB+1]
-    int32_t* cand;      // [nbeam][3 nf]
+    int2* cand;         // [nbeam][3 nf] (blocker facet, f_off of it)
     int32_t* mlist;     // [nbeam][nf]
-    double* recs;       // [nbeam][rs] blocker records, facet j at 2 j + 4 f_off[j]
-    int64_t rs;         // doubles per beam of recs: 2 nf + 4 nvref
     int64_t nvref;      // f_off[nf]
     int64_t b0;         // first beam of this launch
 };
@@ -75,21 +74,40 @@ __device__ __forceinline__ int64_t clampi(int64_t i, int64_t n) { return i < 0 ?
 
 // minloc over the box midpoints of sqrt((bx - px)**2 + (by - py)**2), first minimum
 // (beam_loop_mod.f90:2087-2093)
-__device__ __forceinline__ int nearest_box(const double* bx, const double* by, double px, double py)
+// The square root is monotonic, so the smallest distance is the root of the smallest
+// squared distance d2[k0] (k0 its first index); an earlier box ties with it only if its
+// squared distance rounds to the same root, which needs d2[k] <= d2[k0] (1 + 2^-51): those
+// few are decided by the roots themselves, as the reference does.  bx, by: the box
+// midpoints in registers; gbx, gby: the same in memory.
+__device__ __forceinline__ int nearest_box(const double* bx, const double* by, const double* gbx,
+                                           const double* gby, double px, double py)
 {
 #pragma clang fp contract(off)
-    int best = 0;
-    double dbest = 0.0;
+    double d2[NB];
+    double dmin = 0.0;
+    int k0 = 0;
 #pragma unroll
     for (int k = 0; k < NB; k++) {
         const double dx = bx[k] - px, dy = by[k] - py;
-        const double d = sqrt(dx * dx + dy * dy);
-        if (k == 0 || d < dbest) {
-            dbest = d;
-            best = k;
+        d2[k] = dx * dx + dy * dy;
+        if (k == 0 || d2[k] < dmin) {
+            dmin = d2[k];
+            k0 = k;
         }
     }
-    return best;
+    const double lim = dmin + ldexp(dmin, -49);
+    bool near = false;
+#pragma unroll
+    for (int k = 0; k < NB; k++) near |= k < k0 && d2[k] <= lim;
+    if (!near) return k0;
+    // rare: recompute from the boxes in memory (a rolled loop, no register indexing)
+    const double s0 = sqrt(dmin);
+    for (int k = 0; k < k0; k++) {
+        const double dx = gbx[k] - px, dy = gby[k] - py;
+        const double d = dx * dx + dy * dy;
+        if (d <= lim && sqrt(d) == s0) return k;
+    }
+    return k0;
 }
 
 __device__ __forceinline__ double wave_min(double v)
@@ -206,13 +224,13 @@ __global__ __launch_bounds__(BLK) void k_vis_facets(abt_vis_batch a, Work w)
     const double* x = a.v + 3 * a.nv * b;
     const double* y = x + a.nv;
     const double* mid = a.f_mid + 3 * nf * b + 3 * i;
-    const double mx = mid[0], my = mid[1], mz = mid[2];
-    const int f3 = nearest_box(bx, by, mx, my);
-    const int64_t o0 = a.f_off[i], nvi = a.f_off[i + 1] - o0;
+    const double mx = mid[0], my = mid[1];
+    const int f3 = nearest_box(bx, by, B.bx, B.by, mx, my);
+    const int64_t o0 = a.f_off[i];
     int mask = 0;
     for (int q = 0; q < 3; q++) {
         const int64_t vq = clampi(a.f_vi[min(o0 + q, w.nvref - 1)] - 1, a.nv);
-        mask |= 1 << nearest_box(bx, by, x[vq], y[vq]);
+        mask |= 1 << nearest_box(bx, by, B.bx, B.by, x[vq], y[vq]);
     }
     const double nz = a.f_nz[nf * b + i];
     const int64_t api = a.f_ap[i];
@@ -237,19 +255,6 @@ __global__ __launch_bounds__(BLK) void k_vis_facets(abt_vis_batch a, Work w)
     // the incident beam's blockers face up (:2553)
     w.fmask[o] = (a.mode == ABT_VIS_INC && nz < C001) ? 0 : mask;
     w.mapping[o] = 0;
-    double* rec = w.recs + w.rs * b + 2 * i + 4 * o0;
-    rec[0] = mz;
-    const int2 an = make_int2((int)api, (int)nvi);
-    rec[1] = __builtin_bit_cast(double, an);
-    for (int64_t k = 0; k < nvi; k++) {
-        const int64_t va = clampi(a.f_vi[o0 + k] - 1, a.nv);
-        const int64_t vb = clampi(a.f_vi[k == nvi - 1 ? o0 : o0 + k + 1] - 1, a.nv);
-        double* e = rec + 2 + 4 * k;
-        e[0] = x[va];
-        e[1] = y[va];
-        e[2] = y[vb] - y[va];        // edge_norm1 (:2150-2154)
-        e[3] = -x[vb] + x[va];       // edge_norm2
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -306,8 +311,9 @@ __global__ __launch_bounds__(BLK) void k_vis_lists(abt_vis_batch a, Work w)
         offs[2 * NB + 1] = t;
     }
     __syncthreads();
-    int32_t* cand = w.cand + 3 * nf * b;
+    int2* cand = w.cand + 3 * nf * b;
     int32_t* mlist = w.mlist + nf * b;
+    const int64_t* f_off = a.f_off;
     for (int64_t base = 0; base < nf; base += BLK) {
         const int64_t i = base + tid;
         const bool ok = i < nf;
@@ -325,7 +331,7 @@ __global__ __launch_bounds__(BLK) void k_vis_lists(abt_vis_batch a, Work w)
             if ((bits >> q) & 1u) {
                 int pos = s_run[q] + pre[q];
                 for (int s = 0; s < wv; s++) pos += s_wt[s][q];
-                if (q < NB) cand[pos] = (int32_t)i;
+                if (q < NB) cand[pos] = make_int2((int)i, (int)f_off[i]);
                 else mlist[pos] = (int32_t)i;
             }
         }
@@ -367,59 +373,114 @@ __global__ __launch_bounds__(WAVE) void k_vis_main(abt_vis_batch a, Work w)
     const double* mid = a.f_mid + 3 * nf * b + 3 * m;
     const double mx = mid[0], my = mid[1], mz = mid[2];
     const int apm = (int)a.f_ap[m];
-    const int32_t* cand = w.cand + 3 * nf * b;
+    const int2* cand = w.cand + 3 * nf * b;
     const int32_t* mapping = w.mapping + nf * b;
-    const double* R = w.recs + w.rs * b;
     bool done = !has;
     bool vis = true, inb = MODE == ABT_VIS_INC, shad = false;
     int32_t idb = 0;
     double dist = 0.0, zbest = 0.0;
     uint64_t npair = 0, nedge = 0;
-    const int k1 = offs[bb + 1];
-    for (int k = offs[bb]; k < k1; k++) {
+    // Candidates are staged TC at a time: lane c < TC gathers candidate c's depth, aperture,
+    // vertex count, mapping and first NVS projected vertices into LDS with independent loads;
+    // then every lane walks the tile in order, reading each slot at one LDS address (a
+    // broadcast) and forming the edge normals per pair as the reference does.
+    __shared__ double tile[TC * SLOT];
+    const double* vx = a.v + 3 * a.nv * b;
+    const double* vy = vx + a.nv;
+    const double* zmid = a.f_mid + 3 * nf * b + 2;
+    const int64_t* fvi = a.f_vi;
+    const int k0 = offs[bb], k1 = offs[bb + 1];
+    for (int kt = k0; kt < k1; kt += TC) {
         if (__all(done)) break;
-        const int64_t j = cand[k];
-        if (done || j == m) continue;
-        const double* r = R + 2 * j + 4 * a.f_off[j];
-        const double jz = r[0];
-        if (COUNT) npair++;
-        if (mz > jz) continue;                                    // blocker behind m
-        const int2 an = __builtin_bit_cast(int2, r[1]);
-        bool within = true;
-        for (int e = 0; e < an.y; e++) {
-            const double* q = r + 2 + 4 * e;
-            const double vb1 = mx - q[0], vb2 = my - q[1];
-            const double chk = MODE == ABT_VIS_EXT ? -vb1 * q[2] - vb2 * q[3] : vb1 * q[2] + vb2 * q[3];
-            if (chk > 0) {
-                within = false;
-                break;
+        const int nt = min(TC, k1 - kt);
+        if (lane < nt) {
+            const int2 cj = cand[kt + lane];
+            const int64_t j = cj.x, o0 = cj.y;
+            const int nvj = (int)(a.f_off[j + 1] - o0);
+            double* sl = tile + lane * SLOT;
+            sl[0] = zmid[3 * j];
+            sl[1] = __builtin_bit_cast(double, make_int2((int)a.f_ap[j], nvj));
+            sl[2] = __builtin_bit_cast(double, make_int2((int)j, mapping[j]));
+            sl[3] = __builtin_bit_cast(double, make_int2((int)o0, 0));
+            const int ns = min(nvj, NVS);
+            for (int e = 0; e < ns; e++) {
+                const int64_t vi = clampi(fvi[min(o0 + e, w.nvref - 1)] - 1, a.nv);
+                sl[4 + 2 * e] = vx[vi];
+                sl[5 + 2 * e] = vy[vi];
             }
         }
-        if (COUNT) nedge += (uint64_t)an.y;
-        if (!within) continue;
-        if (MODE == ABT_VIS_INC) {                                // :2573-2580
-            if (apm == an.x) shad = true;
-            else {
+        __syncthreads();
+        for (int c = 0; c < nt; c++) {
+            if (__all(done)) break;
+            const double* sl = tile + c * SLOT;
+            const int2 jm = __builtin_bit_cast(int2, sl[2]);
+            const double jz = sl[0];
+            const int2 an = __builtin_bit_cast(int2, sl[1]);
+            const double x0 = sl[4], y0 = sl[5], x1 = sl[6], y1 = sl[7], x2 = sl[8], y2 = sl[9];
+            if (done || jm.x == m) continue;
+            if (COUNT) npair++;
+            if (mz > jz) continue;                                    // blocker behind m
+            // edge k from vertex k to k+1 (the last back to the first): edge_norm1 = y_{k+1} - y_k,
+            // edge_norm2 = -x_{k+1} + x_k, offset (mx - x_k, my - y_k) (:2149-2162)
+            auto outside = [&](double xa, double ya, double xb, double yb) {
+                const double en2 = -xb + xa, en1 = yb - ya;
+                const double vb1 = mx - xa, vb2 = my - ya;
+                const double chk = MODE == ABT_VIS_EXT ? -vb1 * en1 - vb2 * en2 : vb1 * en1 + vb2 * en2;
+                return chk > 0;
+            };
+            bool within;
+            if (an.y == 3) {
+                const bool o0 = outside(x0, y0, x1, y1), o1 = outside(x1, y1, x2, y2), o2 = outside(x2, y2, x0, y0);
+                within = !(o0 || o1 || o2);
+            } else {
+                within = true;
+                const int64_t o0 = __builtin_bit_cast(int2, sl[3]).x;
+                double xa = x0, ya = y0;
+                for (int e = 0; e < an.y; e++) {
+                    const int en = e + 1 == an.y ? 0 : e + 1;
+                    double xb, yb;
+                    if (en < NVS) {
+                        xb = sl[4 + 2 * en];
+                        yb = sl[5 + 2 * en];
+                    } else {
+                        const int64_t vi = clampi(fvi[min(o0 + en, w.nvref - 1)] - 1, a.nv);
+                        xb = vx[vi];
+                        yb = vy[vi];
+                    }
+                    if (outside(xa, ya, xb, yb)) {
+                        within = false;
+                        break;
+                    }
+                    xa = xb;
+                    ya = yb;
+                }
+            }
+            if (COUNT) nedge += (uint64_t)an.y;
+            if (!within) continue;
+            if (MODE == ABT_VIS_INC) {                                // :2573-2580
+                if (apm == an.x) shad = true;
+                else {
+                    vis = false;
+                    inb = false;
+                }
+                continue;
+            }
+            if (jm.y > 0) {                                           // a beam facet: keep the nearest (:2166-2180)
+                if (!inb || jz < zbest) {
+                    inb = true;
+                    idb = jm.y;
+                    dist = jz - mz;
+                    zbest = jz;
+                }
+            } else {                                                  // a blocker (:2181-2197)
+                if (apm == an.x) continue;
+                if (inb && jz > zbest) continue;
                 vis = false;
                 inb = false;
+                done = true;
             }
-            continue;
         }
-        const int32_t mj = mapping[j];
-        if (mj > 0) {                                             // a beam facet: keep the nearest (:2166-2180)
-            if (!inb || jz < zbest) {
-                inb = true;
-                idb = mj;
-                dist = jz - mz;
-                zbest = jz;
-            }
-        } else {                                                  // a blocker (:2181-2197)
-            if (apm == an.x) continue;
-            if (inb && jz > zbest) continue;
-            vis = false;
-            inb = false;
-            done = true;
-        }
+        __syncthreads();
     }
     if (COUNT) {
 #pragma unroll
@@ -461,7 +522,7 @@ constexpr size_t ALIGN = 256;
 size_t up(size_t n) { return (n + ALIGN - 1) / ALIGN * ALIGN; }
 
 struct Layout {
-    size_t boxes, fbox, fmask, mapping, offs, cand, mlist, recs, total;
+    size_t boxes, fbox, fmask, mapping, offs, cand, mlist, total;
 };
 
 Layout layout(int64_t nbeam, int64_t nf, int64_t nvref)
@@ -474,9 +535,8 @@ Layout layout(int64_t nbeam, int64_t nf, int64_t nvref)
     L.fmask = o;   o += up(nb * f * 4);
     L.mapping = o; o += up(nb * f * 4);
     L.offs = o;    o += up(nb * abt::vis::OFFS * 4);
-    L.cand = o;    o += up(nb * 3 * f * 4);
+    L.cand = o;    o += up(nb * 3 * f * 8);
     L.mlist = o;   o += up(nb * f * 4);
-    L.recs = o;    o += up(nb * (2 * f + 4 * (size_t)nvref) * 8);
     L.total = o;
     return L;
 }
@@ -491,20 +551,22 @@ int launch_all(const abt_vis_batch& a, int64_t nvref, char* ws, hipStream_t st)
     w.fmask = (int32_t*)(ws + L.fmask);
     w.mapping = (int32_t*)(ws + L.mapping);
     w.offs = (int32_t*)(ws + L.offs);
-    w.cand = (int32_t*)(ws + L.cand);
+    w.cand = (int2*)(ws + L.cand);
     w.mlist = (int32_t*)(ws + L.mlist);
-    w.recs = (double*)(ws + L.recs);
-    w.rs = 2 * a.nf + 4 * nvref;
     w.nvref = nvref;
     const unsigned ftiles = (unsigned)((a.nf + BLK - 1) / BLK);
     const unsigned units = (unsigned)((a.nf + WAVE - 1) / WAVE + NB);
+    const bool setup = a.phases != 2, scan = a.phases != 1;
     for (int64_t b0 = 0; b0 < a.nbeam; b0 += BEAM_CHUNK) {
         const unsigned nb = (unsigned)std::min<int64_t>(BEAM_CHUNK, a.nbeam - b0);
         w.b0 = b0;
-        hipLaunchKernelGGL(k_vis_boxes, dim3(nb), dim3(BLK), 0, st, a, w);
-        hipLaunchKernelGGL(k_vis_facets, dim3(ftiles, nb), dim3(BLK), 0, st, a, w);
-        if (a.mode != ABT_VIS_INC) hipLaunchKernelGGL(k_vis_map, dim3(nb), dim3(BLK), 0, st, a, w);
-        hipLaunchKernelGGL(k_vis_lists, dim3(nb), dim3(BLK), 0, st, a, w);
+        if (setup) {
+            hipLaunchKernelGGL(k_vis_boxes, dim3(nb), dim3(BLK), 0, st, a, w);
+            hipLaunchKernelGGL(k_vis_facets, dim3(ftiles, nb), dim3(BLK), 0, st, a, w);
+            if (a.mode != ABT_VIS_INC) hipLaunchKernelGGL(k_vis_map, dim3(nb), dim3(BLK), 0, st, a, w);
+            hipLaunchKernelGGL(k_vis_lists, dim3(nb), dim3(BLK), 0, st, a, w);
+        }
+        if (!scan) continue;
         const dim3 g(units, nb);
 #define VM(M)                                                                           \
     do {                                                                                \
@@ -534,6 +596,7 @@ int check_batch(const abt_vis_batch* a)
         return abt_fail(ABT_ERR_ARG, "visibility: null array");
     if (a->mode != ABT_VIS_INC && !a->field_in) return abt_fail(ABT_ERR_ARG, "visibility: null field_in");
     if (a->mode == ABT_VIS_INC && !a->beam_mid_z) return abt_fail(ABT_ERR_ARG, "find_vis_inc needs beam_mid_z");
+    if (a->phases < 0 || a->phases > 3) return abt_fail(ABT_ERR_ARG, "visibility: phases must be 0..3");
     return 0;
 }
 
@@ -552,6 +615,7 @@ extern "C" int abt_find_vis_dev(const abt_vis_batch* a, int64_t nvref, void* wor
     if (rc) return rc;
     if (a->nbeam == 0) return 0;
     if (nvref < 3 * a->nf) return abt_fail(ABT_ERR_ARG, "visibility: nvref %lld < 3 nf", (long long)nvref);
+    if (nvref >= (1LL << 31)) return abt_fail(ABT_ERR_ARG, "visibility: too many vertex references");
     const size_t need = abt_find_vis_workspace(a->nbeam, a->nf, nvref);
     if (!workspace || bytes < need)
         return abt_fail(ABT_ERR_ARG, "visibility workspace of %zu bytes is smaller than %zu", bytes, need);
@@ -583,6 +647,7 @@ extern "C" int abt_find_vis(int mode, const abt_vis_geometry* geo, int64_t nf_in
                             (long long)(i + 1), (long long)geo->f_ap[i], (long long)na);
     }
     const int64_t nvref = geo->f_off[nf];
+    if (nvref >= (1LL << 31)) return abt_fail(ABT_ERR_ARG, "abt_find_vis: too many vertex references");
     for (int64_t k = 0; k < nvref; k++)
         if (geo->f_vi[k] < 1 || geo->f_vi[k] > nv)
             return abt_fail(ABT_ERR_ARG, "abt_find_vis: vertex id %lld out of 1..%lld", (long long)geo->f_vi[k],
@@ -645,6 +710,7 @@ extern "C" int abt_find_vis(int mode, const abt_vis_geometry* geo, int64_t nf_in
     a.id_beam = (int64_t*)(d + o_id);
     a.is_shad = (uint8_t*)(d + o_shad);
     a.counts = nullptr;
+    a.phases = 0;
     rc = launch_all(a, nvref, d + o_ws, st);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(in_beam, d + o_inb, nf, hipMemcpyDeviceToHost, st));

@@ -35,7 +35,8 @@ class _Batch(ctypes.Structure):
                 ("nf", ctypes.c_int64), ("na", ctypes.c_int64)] + \
                [(n, ctypes.c_void_p) for n in ("f_off", "f_vi", "f_ap", "v", "f_mid", "f_nz", "ap_nz",
                                               "in_off", "field_in", "beam_mid_z", "in_beam", "is_vis",
-                                              "dist_beam", "id_beam", "is_shad", "counts")]
+                                              "dist_beam", "id_beam", "is_shad", "counts")] + \
+               [("phases", ctypes.c_int)]
 
 
 _bound = False
@@ -98,18 +99,15 @@ class VisGeometry:
 
     @staticmethod
     def _derive(verts, off, vi, ap, ap_normals):
-        nf = len(off) - 1
-        mid = np.empty((nf, 3))
-        nz = np.empty(nf)
-        for i in range(nf):
-            ids = vi[off[i]:off[i + 1]] - 1
-            s = np.zeros(3)
-            for k in ids:
-                s = s + verts[k]
-            mid[i] = s / len(ids)
-            p = verts[ids[:3]]
-            c = np.cross(p[1] - p[0], p[2] - p[1])
-            nz[i] = c[2] / np.sqrt(c @ c)
+        nvs = np.diff(off)
+        s = np.zeros((len(nvs), 3))
+        for k in range(int(nvs.max())):          # vertex sums in vertex order
+            has = k < nvs
+            s[has] = s[has] + verts[vi[off[:-1][has] + k] - 1]
+        mid = s / nvs[:, None]
+        p0, p1, p2 = (verts[vi[off[:-1] + q] - 1] for q in range(3))
+        c = np.cross(p1 - p0, p2 - p1)
+        nz = c[:, 2] / np.sqrt((c * c).sum(1))
         g = VisGeometry(verts, off, vi, mid, nz, ap, ap_normals[:, 2].copy())
         g._ap_normals = ap_normals
         return g
@@ -196,18 +194,19 @@ class VisBatch:
         self.work = torch.empty(ws // 8 + 64, dtype=torch.float64, device=dev)
         self.counts = torch.zeros(2, dtype=torch.int64, device=dev)
 
-    def _struct(self, counts):
+    def _struct(self, counts, phases):
         p = lambda t: None if t is None else t.data_ptr()  # noqa: E731
         return _Batch(self.mode, self.nbeam, self.nv, self.nf, self.na, p(self.f_off), p(self.f_vi),
                       p(self.f_ap), p(self.v), p(self.mid), p(self.nz), p(self.ap_nz), p(self.in_off),
                       p(self.field_in), p(self.beam_mid_z), p(self.in_beam), p(self.is_vis),
                       p(self.dist_beam), p(self.id_beam), p(self.is_shad),
-                      p(self.counts) if counts else None)
+                      p(self.counts) if counts else None, phases)
 
-    def run(self, stream=None, counts=False):
+    def run(self, stream=None, counts=False, phases=0):
         """Launch on `stream` (default: torch's current stream).  counts=True adds the
-        reference's work (candidate pairs at the depth test, blocker edges) to self.counts."""
-        b = self._struct(counts)
+        reference's work (candidate pairs at the depth test, blocker edges) to self.counts;
+        phases 1 / 2 run only the set-up / only the blocker scan (0: both)."""
+        b = self._struct(counts, phases)
         wp = self.work.data_ptr()
         wp_al = (wp + 255) // 256 * 256
         nbytes = self.work.numel() * 8 - (wp_al - wp)

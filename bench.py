@@ -52,6 +52,8 @@ def main():
                     help="run the FP64 plain and FP64 batched comparisons of the orientation average (0: skip)")
     ap.add_argument("--adjoint-orient", type=int, default=64,
                     help="orientations of the adjoint workload (0: skip)")
+    ap.add_argument("--vis-beams", type=int, default=4096,
+                    help="beams of the facet-visibility workload (0: skip)")
     ap.add_argument("--batch-tiles", default=None,
                     help="fixed tile shape of the orientation-batched kernel, e.g. 16x1 (default: autotune)")
     ap.add_argument("--json-out", default=None)
@@ -166,6 +168,9 @@ def main():
                                                          args.batch_tiles)
     if args.adjoint_orient > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_orient, lam, dev)
+    if args.vis_beams > 0:
+        out["visibility"] = visibility_bench(args.vis_beams, max(args.steps, 20), max(args.warmup, 2), dev,
+                                             args.cpu_sample > 0)
     line = json.dumps(out)
     if args.json_out:
         with open(args.json_out, "w") as f:
@@ -437,6 +442,81 @@ def adjoint_bench(norient, lam, dev):
             g64 = grads[0].detach().cpu().numpy()
     out["mixed_vs_fp64_grad_rel"] = float(np.linalg.norm(gm - g64) / np.linalg.norm(g64))
     return out
+
+
+def visibility_bench(nbeam, steps, warmup, dev, with_cpu=True):
+    """find_vis_int for `nbeam` internal beams of the rough column (2112 facets), one random
+    orientation per beam, in one batched evaluation per step; inputs resident on the GPU.
+    Metric: facet visibility results per second (beams x facets / step time).  Roofline of the
+    blocker scan k_vis_main: the reference's FP64 work per evaluation, 1 comparison per
+    candidate pair at the depth test plus 7 FLOP per blocker edge it evaluates (edge normal 2,
+    offset 2, dot product 3), counted by the kernel itself in an untimed pass, over the scan's
+    event-timed duration, against the FP64 vector peak."""
+    import torch
+    from abt_amd import visibility as V
+    from abt_amd.workloads import vis_workload
+    t0 = time.time()
+    wl = vis_workload(nbeam, "int", seed=1)
+    topo, v, mid, nz, apn, ino, fin, bmz = wl
+    log(f"visibility workload: {nbeam} internal beams, {topo.nf} facets ({time.time() - t0:.1f} s to generate)")
+    t = lambda a, dt=torch.float64: torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt)  # noqa: E731
+    B = V.VisBatch("int", topo, t(v), t(mid), t(nz), t(apn), t(ino, torch.int64), t(fin, torch.int64), t(bmz))
+    B.run(counts=True)
+    torch.cuda.synchronize()
+    pairs, edges = (int(x) for x in B.counts.cpu().numpy())
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        B.run()
+    ev = []
+    torch.cuda.synchronize()
+    tw = time.perf_counter()
+    for _ in range(steps):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record(stream)
+        B.run(phases=1)
+        e[1].record(stream)
+        B.run(phases=2)
+        e[2].record(stream)
+        ev.append(e)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - tw) / steps
+    setup_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    scan_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    value = nbeam * topo.nf / dt
+    flop = pairs + 7 * edges
+    ach = flop / (scan_ms * 1e-3)
+    log(f"visibility: {dt * 1e3:.3f} ms/step (set-up {setup_ms:.3f} ms, scan {scan_ms:.3f} ms), "
+        f"{value:.4e} facet results/s; {pairs} pairs, {edges} edges")
+    out = {"metric": "facet visibility results per second (find_vis_int)", "value": value,
+           "unit": "facet/s", "ms_per_step": dt * 1e3, "steps": steps, "warmup": warmup, "dtype": "f64+int",
+           "config": {"workload": f"find_vis_int, {nbeam} internal beams of the Gaussian-rough hexagonal column "
+                                  f"(r 10 um, l 20 um, sigma 0.5 um, {topo.nf} facets), one uniform random "
+                                  "orientation per beam (seed 1), illuminating facets: one up-facing aperture",
+                      "beams": nbeam, "facets": topo.nf, "candidate_pairs": pairs, "blocker_edges": edges},
+           "setup_ms": setup_ms, "scan_ms": scan_ms,
+           "roofline": {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s",
+                        "frac": ach / PEAK["fp64"], "traffic": _traffic("k_vis_main<0,false>"),
+                        "kernel": "k_vis_main<0,false>", "kernel_ms": scan_ms, "flop_per_launch": flop}}
+    if with_cpu:
+        out["cpu_baseline"] = vis_cpu_baseline(wl, min(nbeam, 1024), value)
+    return out
+
+
+def vis_cpu_baseline(wl, nsample, gpu_value):
+    """The visibility oracle (oracle/vis_oracle.c, a C restatement of find_vis_int) on the
+    host, one thread, over the first `nsample` beams of the same workload."""
+    import oracle as O
+    topo, v, mid, nz, apn, ino, fin, bmz = wl
+    n = nsample
+    sub = (topo, v[:n], mid[:n], nz[:n], apn[:n], ino[:n + 1], fin[:ino[n]], bmz[:n])
+    t = time.perf_counter()
+    O.find_vis_batch("int", *sub, nthreads=1)
+    dt = time.perf_counter() - t
+    val = n * topo.nf / dt
+    log(f"visibility cpu baseline: {n} beams in {dt:.2f} s, {val:.4e} facet/s (1 thread)")
+    return {"value": val, "unit": "facet/s", "cores": 1, "kind": "port",
+            "sample": f"the first {n} beams of the same workload, find_vis_int restated in C, 1 thread",
+            "gpu_over_cpu": gpu_value / val}
 
 
 def cpu_baseline(beams, th, ph, lam, nsample):

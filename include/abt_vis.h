@@ -74,6 +74,9 @@ int abt_find_vis(int mode, const abt_vis_geometry* geo, int64_t nf_in, const int
  * counts: NULL, or 2 device uint64 the launch adds its work to: [0] candidate
  * (facet, blocker) pairs that reach the depth test, [1] blocker edges the
  * reference evaluates for them (nv of the blocker per pair passing the test).
+ * phases: 0 (or 3) runs the whole evaluation; 1 runs only the set-up (boxes, flags,
+ * candidate lists, blocker records), 2 only the blocker scan, which then reads the
+ * set-up that the last phase-1 call left in the same workspace for the same inputs.
  * The workspace (abt_find_vis_workspace bytes, 256-byte aligned) is the caller's;
  * launches that share one must not overlap. */
 typedef struct abt_vis_batch {
@@ -95,6 +98,7 @@ typedef struct abt_vis_batch {
     int64_t* id_beam;
     uint8_t* is_shad;
     unsigned long long* counts;
+    int phases;
 } abt_vis_batch;
 
 /* Device workspace bytes for nbeam beams of nf facets with nvref = f_off[nf] vertex references. */

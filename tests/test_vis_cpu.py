@@ -80,8 +80,8 @@ def test_workspace_grows_with_beams_and_facets():
     L = V._lib()
     w1 = L.abt_find_vis_workspace(1, 2112, 3 * 2112)
     w2 = L.abt_find_vis_workspace(10, 2112, 3 * 2112)
-    # per beam: records (2 nf + 4 nvref doubles), lists and flags (7 nf int32), box data
-    assert w1 >= 8 * (2 * 2112 + 12 * 2112) + 4 * 7 * 2112
+    # per beam: flags (3 nf int32), candidate lists (3 nf int2), tested-facet lists (nf int32), box data
+    assert w1 >= (3 * 4 + 3 * 8 + 4) * 2112
     assert w2 >= 10 * (w1 - 8 * 256) and w2 % 256 == 0
 
 
