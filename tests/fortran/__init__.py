@@ -54,3 +54,46 @@ def run(theta, phi, lam, verts, faces, b_fi, b_ampl, b_vk7, b_prop, b_fov, e_fi,
     out.update({k: parts[k] for k in ("x", "y", "z")})
     out["log"] = open(flog).read()
     return out
+
+
+VIS_DRIVER = os.path.join(HERE, "_build", "abt_vis_driver")
+
+
+def run_vis(g, field_in, beam_mid_z, workdir=None):
+    """Runs find_vis_int, find_vis_ext and find_vis_inc of the Fortran drop-in vis_mod on the
+    geometry g (abt_amd.visibility.VisGeometry, facet normals given only by their z part: the
+    x, y parts are written as 0) with the 1-based illuminating facets field_in.  Returns a
+    dict per mode of in_beam, is_vis (inc only), dist_beam, id_beam, is_shad."""
+    workdir = workdir or tempfile.mkdtemp(prefix="abt_v_")
+    fin = os.path.join(workdir, "vin.bin")
+    fout = os.path.join(workdir, "vout.bin")
+    nf = g.nf
+    with open(fin, "wb") as f:
+        np.array([g.nv, nf, g.na, len(field_in)], np.int64).tofile(f)
+        np.asarray(g.v, np.float64).T.copy().tofile(f)          # v(nv, 3), column-major
+        np.diff(g.f_off).astype(np.int64).tofile(f)
+        np.asarray(g.f_vi, np.int64).tofile(f)
+        np.asarray(g.mid, np.float64).tofile(f)                 # mid(3, nf)
+        n = np.zeros((nf, 3))
+        n[:, 2] = g.nz
+        n.T.copy().tofile(f)                                    # n(nf, 3)
+        np.asarray(g.ap, np.int64).tofile(f)
+        a = np.zeros((g.na, 3))
+        a[:, 2] = g.ap_nz
+        a.T.copy().tofile(f)                                    # ap normals (na, 3)
+        np.asarray(field_in, np.int64).tofile(f)
+        np.array([beam_mid_z], np.float64).tofile(f)
+    subprocess.run([VIS_DRIVER, fin, fout], check=True, timeout=600)
+    raw = open(fout, "rb").read()
+    out, off = {}, 0
+    for mode in ("int", "ext", "inc"):
+        r = {}
+        for key, dt in (("in_beam", np.int8), ("is_vis", np.int8), ("dist_beam", np.float64),
+                        ("id_beam", np.int64), ("is_shad", np.int8)):
+            n_b = nf * np.dtype(dt).itemsize
+            r[key] = np.frombuffer(raw[off:off + n_b], dtype=dt).copy()
+            off += n_b
+        for key in ("in_beam", "is_vis", "is_shad"):
+            r[key] = r[key] != 0
+        out[mode] = r
+    return out
