@@ -46,6 +46,7 @@ constexpr int BLK = 256;
 constexpr int WAVE = 64;
 constexpr int OFFS = 2 * (NB + 1) + 2;                // per-beam list offsets (padded)
 constexpr int TC = 16;                                // candidates per LDS tile of the scan
+constexpr int UNR = 4;                                // candidates evaluated together in the scan
 constexpr int NVS = 8;                                // vertices per candidate staged in LDS
 constexpr int SLOT = 4 + 2 * NVS;                     // doubles per staged candidate
 constexpr int64_t BEAM_CHUNK = 65535;                 // beams per launch (grid.y)
@@ -410,74 +411,100 @@ __global__ __launch_bounds__(WAVE) void k_vis_main(abt_vis_batch a, Work w)
             }
         }
         __syncthreads();
-        for (int c = 0; c < nt; c++) {
+        // UNR candidates at a time: their depth and edge tests are independent of the scan state,
+        // so they are evaluated together (their LDS reads and FP64 chains overlap); the
+        // reference's order-dependent rules are then applied to them in candidate order.
+        for (int c = 0; c < nt; c += UNR) {
             if (__all(done)) break;
-            const double* sl = tile + c * SLOT;
-            const int2 jm = __builtin_bit_cast(int2, sl[2]);
-            const double jz = sl[0];
-            const int2 an = __builtin_bit_cast(int2, sl[1]);
-            const double x0 = sl[4], y0 = sl[5], x1 = sl[6], y1 = sl[7], x2 = sl[8], y2 = sl[9];
-            if (done || jm.x == m) continue;
-            if (COUNT) npair++;
-            if (mz > jz) continue;                                    // blocker behind m
-            // edge k from vertex k to k+1 (the last back to the first): edge_norm1 = y_{k+1} - y_k,
-            // edge_norm2 = -x_{k+1} + x_k, offset (mx - x_k, my - y_k) (:2149-2162)
-            auto outside = [&](double xa, double ya, double xb, double yb) {
-                const double en2 = -xb + xa, en1 = yb - ya;
-                const double vb1 = mx - xa, vb2 = my - ya;
-                const double chk = MODE == ABT_VIS_EXT ? -vb1 * en1 - vb2 * en2 : vb1 * en1 + vb2 * en2;
-                return chk > 0;
-            };
-            bool within;
-            if (an.y == 3) {
-                const bool o0 = outside(x0, y0, x1, y1), o1 = outside(x1, y1, x2, y2), o2 = outside(x2, y2, x0, y0);
-                within = !(o0 || o1 || o2);
-            } else {
-                within = true;
-                const int64_t o0 = __builtin_bit_cast(int2, sl[3]).x;
-                double xa = x0, ya = y0;
-                for (int e = 0; e < an.y; e++) {
-                    const int en = e + 1 == an.y ? 0 : e + 1;
-                    double xb, yb;
-                    if (en < NVS) {
-                        xb = sl[4 + 2 * en];
-                        yb = sl[5 + 2 * en];
-                    } else {
-                        const int64_t vi = clampi(fvi[min(o0 + en, w.nvref - 1)] - 1, a.nv);
-                        xb = vx[vi];
-                        yb = vy[vi];
+            bool pr[UNR], fr[UNR], hit[UNR];
+            double qz[UNR];
+            int qap[UNR], qmap[UNR], qnv[UNR];
+#pragma unroll
+            for (int q = 0; q < UNR; q++) {
+                const bool valid = c + q < nt;
+                const double* sl = tile + (valid ? c + q : 0) * SLOT;
+                const int2 jm = __builtin_bit_cast(int2, sl[2]);
+                const int2 an = __builtin_bit_cast(int2, sl[1]);
+                qz[q] = sl[0];
+                qap[q] = an.x;
+                qnv[q] = an.y;
+                qmap[q] = jm.y;
+                pr[q] = valid && jm.x != m;
+                fr[q] = pr[q] && !(mz > qz[q]);                         // blocker not behind m
+                // edge k from vertex k to k+1 (the last back to the first): edge_norm1 =
+                // y_{k+1} - y_k, edge_norm2 = -x_{k+1} + x_k, offset (mx - x_k, my - y_k)
+                // (:2149-2162)
+                auto outside = [&](double xa, double ya, double xb, double yb) {
+                    const double en2 = -xb + xa, en1 = yb - ya;
+                    const double vb1 = mx - xa, vb2 = my - ya;
+                    const double chk = MODE == ABT_VIS_EXT ? -vb1 * en1 - vb2 * en2 : vb1 * en1 + vb2 * en2;
+                    return chk > 0;
+                };
+                const double x0 = sl[4], y0 = sl[5];
+                bool within;
+                if (an.y == 3) {
+                    const double x1 = sl[6], y1 = sl[7], x2 = sl[8], y2 = sl[9];
+                    const bool o0 = outside(x0, y0, x1, y1), o1 = outside(x1, y1, x2, y2),
+                               o2 = outside(x2, y2, x0, y0);
+                    within = !(o0 || o1 || o2);
+                } else {
+                    within = true;
+                    const int64_t o0 = __builtin_bit_cast(int2, sl[3]).x;
+                    double xa = x0, ya = y0;
+                    for (int e = 0; e < an.y; e++) {
+                        const int en = e + 1 == an.y ? 0 : e + 1;
+                        double xb, yb;
+                        if (en < NVS) {
+                            xb = sl[4 + 2 * en];
+                            yb = sl[5 + 2 * en];
+                        } else {
+                            const int64_t vi = clampi(fvi[min(o0 + en, w.nvref - 1)] - 1, a.nv);
+                            xb = vx[vi];
+                            yb = vy[vi];
+                            // keeps the two branches' loads apart: merged, they would become one
+                            // load through a generic pointer
+                            asm volatile("" : "+v"(xb), "+v"(yb));
+                        }
+                        if (outside(xa, ya, xb, yb)) {
+                            within = false;
+                            break;
+                        }
+                        xa = xb;
+                        ya = yb;
                     }
-                    if (outside(xa, ya, xb, yb)) {
-                        within = false;
-                        break;
-                    }
-                    xa = xb;
-                    ya = yb;
                 }
+                hit[q] = fr[q] && within;
             }
-            if (COUNT) nedge += (uint64_t)an.y;
-            if (!within) continue;
-            if (MODE == ABT_VIS_INC) {                                // :2573-2580
-                if (apm == an.x) shad = true;
-                else {
+#pragma unroll
+            for (int q = 0; q < UNR; q++) {
+                if (done) break;
+                if (COUNT) {
+                    npair += pr[q] ? 1 : 0;
+                    nedge += fr[q] ? (uint64_t)qnv[q] : 0;
+                }
+                if (!hit[q]) continue;
+                if (MODE == ABT_VIS_INC) {                                // :2573-2580
+                    if (apm == qap[q]) shad = true;
+                    else {
+                        vis = false;
+                        inb = false;
+                    }
+                    continue;
+                }
+                if (qmap[q] > 0) {                                        // a beam facet: keep the nearest (:2166-2180)
+                    if (!inb || qz[q] < zbest) {
+                        inb = true;
+                        idb = qmap[q];
+                        dist = qz[q] - mz;
+                        zbest = qz[q];
+                    }
+                } else {                                                  // a blocker (:2181-2197)
+                    if (apm == qap[q]) continue;
+                    if (inb && qz[q] > zbest) continue;
                     vis = false;
                     inb = false;
+                    done = true;
                 }
-                continue;
-            }
-            if (jm.y > 0) {                                           // a beam facet: keep the nearest (:2166-2180)
-                if (!inb || jz < zbest) {
-                    inb = true;
-                    idb = jm.y;
-                    dist = jz - mz;
-                    zbest = jz;
-                }
-            } else {                                                  // a blocker (:2181-2197)
-                if (apm == an.x) continue;
-                if (inb && jz > zbest) continue;
-                vis = false;
-                inb = false;
-                done = true;
             }
         }
         __syncthreads();
