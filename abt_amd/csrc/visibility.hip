@@ -45,7 +45,7 @@ constexpr int NB = 16;                                // 4 x 4 boxes
 constexpr int BLK = 256;
 constexpr int WAVE = 64;
 constexpr int OFFS = 2 * (NB + 1) + 2;                // per-beam list offsets (padded)
-constexpr int TC = 16;                                // candidates per LDS tile of the scan
+constexpr int TC = 32;                                // candidates per LDS tile of the scan
 constexpr int UNR = 4;                                // candidates evaluated together in the scan
 constexpr int NVS = 8;                                // vertices per candidate staged in LDS
 constexpr int SLOT = 4 + 2 * NVS;                     // doubles per staged candidate
