@@ -6,7 +6,7 @@ facet, including the values the reference leaves behind on blocked facets, and t
 counters equal to the oracle's.  Cases: the rough column (2112 facets) in random beam frames
 for all three modes, the layered two-body scenes of the oracle pins, polygon facets
 (quadrilaterals and hexagons), a repeated illuminating facet, degenerate boxes (every vertex
-on one x), and a full 2048-beam batch.
+on one x), a full 2048-beam batch, and the 20,736-facet column.
 """
 import numpy as np
 import pytest
@@ -158,4 +158,16 @@ def test_full_batch_matches_oracle():
     r, cnt = _batch("int", *args)
     ref, rc = O.find_vis_batch("int", *args, nthreads=16)
     _same(r, ref, "2048 beams")
+    np.testing.assert_array_equal(cnt, rc.sum(0))
+
+
+@pytest.mark.parametrize("mode", ["int", "inc"])
+def test_large_column_matches_oracle(mode):
+    """The 20,736-facet column (radius 50 um, length 100 um): long candidate lists, many
+    tested facets per box."""
+    args = vis_workload(3, mode, particle="large_column", seed=4)
+    print(f"vis large column {mode} 3 beams", flush=True)
+    r, cnt = _batch(mode, *args)
+    ref, rc = O.find_vis_batch(mode, *args, nthreads=3)
+    _same(r, ref, f"large {mode}")
     np.testing.assert_array_equal(cnt, rc.sum(0))
