@@ -14,3 +14,12 @@ def test_oracle_reproduces_golden_small():
     fin = O.finalise(th, ph, lam, rb, re)
     np.testing.assert_array_equal(fin["mueller"], d["mueller"])
     np.testing.assert_array_equal(fin["mueller_1d"], d["mueller_1d"])
+
+
+def test_oracle_reproduces_golden_visibility():
+    from tests.golden import load_vis
+    for name, g, fin, bz, res in load_vis():
+        for mode in ("int", "ext", "inc"):
+            r = O.find_vis(mode, g, None if mode == "inc" else fin, bz)
+            for k, want in res[mode].items():
+                np.testing.assert_array_equal(r[k], want, err_msg=f"{name} {mode} {k}")

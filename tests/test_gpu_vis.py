@@ -171,3 +171,12 @@ def test_large_column_matches_oracle(mode):
     ref, rc = O.find_vis_batch(mode, *args, nthreads=3)
     _same(r, ref, f"large {mode}")
     np.testing.assert_array_equal(cnt, rc.sum(0))
+
+
+def test_golden_fixture():
+    """The committed fixture (tests/golden/vis_small.npz), through the host API."""
+    from tests.golden import load_vis
+    for name, g, fin, bz, res in load_vis():
+        for mode in ("int", "ext", "inc"):
+            print(f"vis golden {name} {mode}", flush=True)
+            _same(V.find_vis(mode, g, None if mode == "inc" else fin, bz), res[mode], f"golden {name} {mode}")
