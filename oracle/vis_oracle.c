@@ -34,7 +34,7 @@
  * Parity rests on checks that do not restate this file: closed forms on convex
  * bodies, brute-force point-in-polygon occlusion on layered scenes, the box
  * construction against its closed form, and planted mistakes
- * (tests/test_oracle_pins.py).
+ * (tests/test_vis_oracle.py).
  */
 #include <math.h>
 #include <stdint.h>
@@ -156,6 +156,11 @@ void oracle_find_vis(int mode, int64_t nv, const double* v, int64_t nf, const in
     for (int64_t i = 0; i < nf_in; i++) {
         is_beam[field_in[i] - 1] = 1;
         mapping[field_in[i] - 1] = i + 1;
+    }
+    if (!inc && nf_in == 0) {
+        /* no illuminating facet: nothing is lit (the reference reads field_in(1) out of
+         * range here; the C-ABI documents this outcome) */
+        for (int64_t i = 0; i < nf; i++) is_vis[i] = 0;
     }
     if (!inc && nf_in > 0) {
         /* beam extent in x-y (beam_loop_mod.f90:2124-2143) and the midpoint cull (:2145-2155) */

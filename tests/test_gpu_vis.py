@@ -180,3 +180,61 @@ def test_golden_fixture():
         for mode in ("int", "ext", "inc"):
             print(f"vis golden {name} {mode}", flush=True)
             _same(V.find_vis(mode, g, None if mode == "inc" else fin, bz), res[mode], f"golden {name} {mode}")
+
+
+def _ngon_prism(k, R):
+    """A prism with k-gon ends (k > 8: the scan reads the vertices past the eighth from
+    memory) next to a triangulated column."""
+    ang = np.arange(k) * 2 * np.pi / k
+    ring = np.stack([3 * np.cos(ang), 3 * np.sin(ang)], 1)
+    v = np.concatenate([np.c_[ring, np.full(k, -1.5)], np.c_[ring, np.full(k, 1.5)]]) + [0.5, -0.5, 4.0]
+    faces = [[s, (s + 1) % k, k + (s + 1) % k, k + s] for s in range(k)]
+    faces += [list(range(k - 1, -1, -1)), list(range(k, 2 * k))]
+    mid = ang + np.pi / k
+    nrm = np.concatenate([np.stack([np.cos(mid), np.sin(mid), np.zeros(k)], 1), [[0, 0, -1.0], [0, 0, 1.0]]])
+    v2, f2, a2, n2 = hex_column_apertures(6.0, 3.0, 3, 2)
+    allf = faces + [list(t + len(v)) for t in f2]
+    ap = np.concatenate([np.arange(1, k + 3), a2 + k + 2])
+    return V.VisGeometry.from_mesh(np.concatenate([v, v2]), allf, ap, np.concatenate([nrm, n2])).rotated(R)
+
+
+@pytest.mark.parametrize("mode", ["int", "ext", "inc"])
+def test_many_vertex_facets_match_oracle(mode):
+    for s, k in enumerate((9, 12)):
+        g = _ngon_prism(k, random_rotations(1, 500 + s)[0])
+        want = g.ap_nz > 0.01 if mode == "int" else g.ap_nz < -0.01
+        f = None
+        if mode != "inc":
+            f = np.nonzero(g.ap == np.nonzero(want)[0][0] + 1)[0] + 1
+        bz = g.v[:, 2].max() + 1.0
+        print(f"vis {k}-gon {mode}", flush=True)
+        _same(V.find_vis(mode, g, f, bz), O.find_vis(mode, g, f, bz), f"{k}-gon {mode}")
+
+
+def test_beams_past_one_launch_and_empty_beams():
+    """70,000 beams of a small column (more than one launch's 65,535 beams), some with no
+    illuminating facets (nothing lit)."""
+    v, f, a, n = hex_column_apertures(2.0, 3.0, 1, 1)
+    topo = V.VisGeometry.from_mesh(v, f, a, n)
+    nb = 70000
+    rots = random_rotations(64, 8)
+    geos = [topo.rotated(R) for R in rots]
+    pick = np.arange(nb) % 64
+    vv = np.stack([geos[i].v.T for i in pick])
+    mid = np.stack([geos[i].mid for i in pick])
+    nz = np.stack([geos[i].nz for i in pick])
+    apn = np.stack([geos[i].ap_nz for i in pick])
+    fins, ino = [], [0]
+    for b in range(nb):
+        g = geos[pick[b]]
+        up = np.nonzero(g.ap_nz > 0.01)[0]
+        fi = np.zeros(0, np.int64) if b % 997 == 0 else np.nonzero(g.ap == up[b % len(up)] + 1)[0] + 1
+        fins.append(fi)
+        ino.append(ino[-1] + len(fi))
+    args = (topo, vv, mid, nz, apn, np.array(ino, np.int64), np.concatenate(fins), np.zeros(nb))
+    print(f"vis batch int {nb} beams", flush=True)
+    r, cnt = _batch("int", *args)
+    ref, rc = O.find_vis_batch("int", *args, nthreads=16)
+    _same(r, ref, f"{nb} beams")
+    np.testing.assert_array_equal(cnt, rc.sum(0))
+    assert not r["in_beam"][::997].any()
