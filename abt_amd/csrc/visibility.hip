@@ -536,6 +536,53 @@ __global__ __launch_bounds__(WAVE) void k_vis_main(abt_vis_batch a, Work w)
     }
 }
 
+// ---------------------------------------------------------------------------
+// rotate (beam_loop_mod.f90:1467-1526) applied to many beams: vertices, facet normals,
+// midpoints and aperture normals each multiplied by the beam's rotation, every element as
+// ((r_i1 p_1 + r_i2 p_2) + r_i3 p_3) like the reference's matmul (contraction off).
+__device__ __forceinline__ double rot_row(const double* R, int i, double px, double py, double pz)
+{
+#pragma clang fp contract(off)
+    double a = R[3 * i] * px;
+    a = a + R[3 * i + 1] * py;
+    a = a + R[3 * i + 2] * pz;
+    return a;
+}
+
+__global__ __launch_bounds__(BLK) void k_vis_rotate(int64_t b0, const double* __restrict__ rot, int64_t nv,
+                                                    const double* __restrict__ v0, int64_t nf,
+                                                    const double* __restrict__ mid0, const double* __restrict__ n0,
+                                                    int64_t na, const double* __restrict__ apn0,
+                                                    double* __restrict__ v, double* __restrict__ mid,
+                                                    double* __restrict__ nz, double* __restrict__ ap_nz)
+{
+    const int64_t b = b0 + blockIdx.y;
+    const int64_t i = (int64_t)blockIdx.x * BLK + threadIdx.x;
+    double R[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) R[k] = rot[9 * b + k];
+    if (i < nv) {
+        const double x = v0[i], y = v0[nv + i], z = v0[2 * nv + i];
+        double* o = v + 3 * nv * b;
+        o[i] = rot_row(R, 0, x, y, z);
+        o[nv + i] = rot_row(R, 1, x, y, z);
+        o[2 * nv + i] = rot_row(R, 2, x, y, z);
+    }
+    if (i < nf) {
+        const double* p = mid0 + 3 * i;
+        double* o = mid + 3 * nf * b + 3 * i;
+        o[0] = rot_row(R, 0, p[0], p[1], p[2]);
+        o[1] = rot_row(R, 1, p[0], p[1], p[2]);
+        o[2] = rot_row(R, 2, p[0], p[1], p[2]);
+        const double* q = n0 + 3 * i;
+        nz[nf * b + i] = rot_row(R, 2, q[0], q[1], q[2]);
+    }
+    if (i < na) {
+        const double* q = apn0 + 3 * i;
+        ap_nz[na * b + i] = rot_row(R, 2, q[0], q[1], q[2]);
+    }
+}
+
 }  // namespace vis
 }  // namespace abt
 
@@ -746,5 +793,70 @@ extern "C" int abt_find_vis(int mode, const abt_vis_geometry* geo, int64_t nf_in
     HIPCHK(hipMemcpyAsync(id_beam, d + o_id, 8 * nf, hipMemcpyDeviceToHost, st));
     HIPCHK(hipMemcpyAsync(is_shad, d + o_shad, nf, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// rotate()'s rotation for each beam (beam_loop_mod.f90:1488-1503), on the host with the C
+// library's atan2, acos, sqrt and sincos like the reference (pi its REAL*4 literal,
+// misc_submod.f90:12): R[9 b + 3 i + k] = rot(i+1, k+1).
+extern "C" int abt_vis_rotations(int64_t nbeam, const double* prop, double* R)
+{
+#pragma clang fp contract(off)
+    if (nbeam < 0 || (nbeam > 0 && (!prop || !R))) return abt_fail(ABT_ERR_ARG, "abt_vis_rotations: bad arguments");
+    const double pi = 3.1415927410125732;
+    for (int64_t b = 0; b < nbeam; b++) {
+        const double* p = prop + 3 * b;
+        double r1[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, r2[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, c, sn;
+        const double th1 = 2 * pi - atan2(p[1], p[0]);
+        sincos(th1, &sn, &c);
+        r1[0] = c;
+        r1[1] = -sn;
+        r1[3] = sn;
+        r1[4] = c;
+        r1[8] = 1;
+        double t[3];
+        for (int i = 0; i < 3; i++) {
+            double a = 0.0;
+            for (int k = 0; k < 3; k++) a = a + r1[3 * i + k] * p[k];
+            t[i] = a;
+        }
+        const double nrm = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);   // normalise_vec (misc_submod.f90:2216-2227)
+        for (int i = 0; i < 3; i++) t[i] = t[i] / nrm;
+        sincos(acos(-t[2]), &sn, &c);
+        r2[0] = c;
+        r2[2] = sn;
+        r2[6] = -sn;
+        r2[8] = c;
+        r2[4] = 1;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double a = 0.0;
+                for (int k = 0; k < 3; k++) a = a + r2[3 * i + k] * r1[3 * k + j];
+                R[9 * b + 3 * i + j] = a;
+            }
+    }
+    return 0;
+}
+
+extern "C" int abt_vis_rotate_dev(int64_t nbeam, const double* rot, int64_t nv, const double* v0, int64_t nf,
+                                  const double* mid0, const double* n0, int64_t na, const double* apn0, double* v,
+                                  double* mid, double* nz, double* ap_nz, void* stream)
+{
+    using namespace abt::vis;
+    if (nbeam < 0 || nv < 1 || nf < 1 || na < 1) return abt_fail(ABT_ERR_ARG, "abt_vis_rotate_dev: bad sizes");
+    if (nbeam == 0) return 0;
+    if (!rot || !v0 || !mid0 || !n0 || !apn0 || !v || !mid || !nz || !ap_nz)
+        return abt_fail(ABT_ERR_ARG, "abt_vis_rotate_dev: null array");
+    int rc = abt_ensure_device(nullptr);
+    if (rc) return rc;
+    const int64_t n = std::max(nv, std::max(nf, na));
+    const unsigned gx = (unsigned)((n + BLK - 1) / BLK);
+    for (int64_t b0 = 0; b0 < nbeam; b0 += BEAM_CHUNK) {
+        const unsigned nb = (unsigned)std::min<int64_t>(BEAM_CHUNK, nbeam - b0);
+        hipLaunchKernelGGL(k_vis_rotate, dim3(gx, nb), dim3(BLK), 0, (hipStream_t)stream, b0, rot, nv, v0, nf, mid0,
+                           n0, na, apn0, v, mid, nz, ap_nz);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return abt_fail(ABT_ERR_HIP, "abt_vis_rotate_dev launch failed: %s", hipGetErrorString(e));
     return 0;
 }

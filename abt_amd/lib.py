@@ -35,7 +35,7 @@ EXPORTS = [
     "abt_batch_workspace", "abt_batch_set_tiles", "abt_batch_get_tiles", "abt_batch_autotune",
     "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_adjoint_plan_create",
     "abt_adjoint_plan_destroy", "abt_adjoint_run", "abt_find_vis", "abt_find_vis_workspace",
-    "abt_find_vis_dev",
+    "abt_find_vis_dev", "abt_vis_rotations", "abt_vis_rotate_dev",
 ]
 
 

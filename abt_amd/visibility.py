@@ -52,6 +52,9 @@ def _lib():
         L.abt_find_vis_workspace.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64]
         L.abt_find_vis_workspace.restype = ctypes.c_size_t
         L.abt_find_vis_dev.argtypes = [ctypes.POINTER(_Batch), ctypes.c_int64, vp, ctypes.c_size_t, vp]
+        i64 = ctypes.c_int64
+        L.abt_vis_rotations.argtypes = [i64, vp, vp]
+        L.abt_vis_rotate_dev.argtypes = [i64, vp, i64, vp, i64, vp, vp, i64, vp, vp, vp, vp, vp, vp]
         _bound = True
     return L
 
@@ -61,7 +64,8 @@ class VisGeometry:
     """The geometry_type fields (types_mod.f90:156-190) the visibility routines read.
 
     v (nv, 3); facets as CSR: f_off (nf+1), f_vi 1-based vertex ids; mid (nf, 3);
-    nz (nf) = n(f%ni, 3); ap (nf) 1-based aperture ids; ap_nz (na) = ap(:)%n(3)."""
+    nz (nf) = n(f%ni, 3); ap (nf) 1-based aperture ids; ap_nz (na) = ap(:)%n(3); optionally
+    the full facet normals n (nf, 3) and aperture normals ap_n (na, 3), which rotations need."""
     v: np.ndarray
     f_off: np.ndarray
     f_vi: np.ndarray
@@ -69,6 +73,8 @@ class VisGeometry:
     nz: np.ndarray
     ap: np.ndarray
     ap_nz: np.ndarray
+    n: np.ndarray = None
+    ap_n: np.ndarray = None
 
     @property
     def nf(self):
@@ -107,16 +113,15 @@ class VisGeometry:
         mid = s / nvs[:, None]
         p0, p1, p2 = (verts[vi[off[:-1] + q] - 1] for q in range(3))
         c = np.cross(p1 - p0, p2 - p1)
-        nz = c[:, 2] / np.sqrt((c * c).sum(1))
-        g = VisGeometry(verts, off, vi, mid, nz, ap, ap_normals[:, 2].copy())
-        g._ap_normals = ap_normals
-        return g
+        n = c / np.sqrt((c * c).sum(1))[:, None]
+        return VisGeometry(verts, off, vi, mid, n[:, 2].copy(), ap, ap_normals[:, 2].copy(), n,
+                           np.asarray(ap_normals, dtype=np.float64))
 
     def rotated(self, R):
         """The geometry seen in a frame rotated by R (vertices, normals, aperture normals,
         midpoints recomputed), as the tracer does before find_vis_int / find_vis_ext."""
         R = np.asarray(R, dtype=np.float64)
-        return VisGeometry._derive(self.v @ R.T, self.f_off, self.f_vi, self.ap, self._ap_normals @ R.T)
+        return VisGeometry._derive(self.v @ R.T, self.f_off, self.f_vi, self.ap, self.ap_n @ R.T)
 
 
 def _host_outputs(nf):
@@ -218,3 +223,33 @@ class VisBatch:
         return dict(in_beam=self.in_beam.view(sh).bool(), is_vis=self.is_vis.view(sh).bool(),
                     dist_beam=self.dist_beam.view(sh), id_beam=self.id_beam.view(sh),
                     is_shad=self.is_shad.view(sh).bool())
+
+
+def vis_rotations(props):
+    """rotate()'s rotation (beam_loop_mod.f90:1488-1503) for each propagation direction:
+    props (nbeam, 3) -> R (nbeam, 3, 3) taking each to -z (host, C library)."""
+    p = np.ascontiguousarray(props, dtype=np.float64).reshape(-1, 3)
+    R = np.empty((len(p), 3, 3))
+    lib._check(_lib().abt_vis_rotations(len(p), p.ctypes.data, R.ctypes.data))
+    return R
+
+
+def rotate_dev(R, geom: VisGeometry, stream=None):
+    """The particle geometry rotated into each beam's frame on the GPU, as rotate() does
+    (beam_loop_mod.f90:1467-1526): R a (nbeam, 3, 3) device tensor; returns device tensors
+    v (nbeam, 3, nv), mid (nbeam, nf, 3), nz (nbeam, nf), ap_nz (nbeam, na), the inputs of
+    VisBatch."""
+    import torch
+    dev = R.device
+    nb = R.shape[0]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)  # noqa: E731
+    v0, mid0, n0, apn0 = t(geom.v.T), t(geom.mid), t(geom.n), t(geom.ap_n)
+    v = torch.empty((nb, 3, geom.nv), dtype=torch.float64, device=dev)
+    mid = torch.empty((nb, geom.nf, 3), dtype=torch.float64, device=dev)
+    nz = torch.empty((nb, geom.nf), dtype=torch.float64, device=dev)
+    apnz = torch.empty((nb, geom.na), dtype=torch.float64, device=dev)
+    Rc = R.contiguous()
+    lib._check(_lib().abt_vis_rotate_dev(nb, Rc.data_ptr(), geom.nv, v0.data_ptr(), geom.nf, mid0.data_ptr(),
+                                         n0.data_ptr(), geom.na, apn0.data_ptr(), v.data_ptr(), mid.data_ptr(),
+                                         nz.data_ptr(), apnz.data_ptr(), lib._stream_ptr(stream)))
+    return v, mid, nz, apnz

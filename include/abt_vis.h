@@ -101,6 +101,19 @@ typedef struct abt_vis_batch {
     int phases;
 } abt_vis_batch;
 
+/* The beam frames of the tracer's rotate() (beam_loop_mod.f90:1467-1526; no counterpart
+ * as a separate routine).  abt_vis_rotations: for nbeam propagation directions prop
+ * [nbeam][3] (host), the rotations R [nbeam][9] (host, R[9 b + 3 i + k] = rot(i+1, k+1))
+ * taking each to -z, computed with the C library as the reference does.
+ * abt_vis_rotate_dev: the particle geometry rotated into each beam's frame (device arrays):
+ * v0 (nv,3) column-major, mid0 [nf][3], n0 [nf][3] (each facet's normal), apn0 [na][3]
+ * -> v [nbeam][3][nv], mid [nbeam][nf][3], nz [nbeam][nf], ap_nz [nbeam][na], the inputs
+ * of abt_find_vis_dev; bit-identical to rotate(). */
+int abt_vis_rotations(int64_t nbeam, const double* prop, double* R);
+int abt_vis_rotate_dev(int64_t nbeam, const double* R, int64_t nv, const double* v0, int64_t nf,
+                       const double* mid0, const double* n0, int64_t na, const double* apn0, double* v,
+                       double* mid, double* nz, double* ap_nz, void* stream);
+
 /* Device workspace bytes for nbeam beams of nf facets with nvref = f_off[nf] vertex references. */
 size_t abt_find_vis_workspace(int64_t nbeam, int64_t nf, int64_t nvref);
 /* nvref must equal f_off[nf] (the host copy of the last offset). */

@@ -274,3 +274,29 @@ def find_vis_batch(mode, topo, v, mid, nz, ap_nz, in_off, field_in, beam_mid_z=N
                                   cnt.ctypes.data, int(nthreads))
     return dict(in_beam=inb.astype(bool), is_vis=vis.astype(bool), dist_beam=dist, id_beam=idb,
                 is_shad=shad.astype(bool)), cnt
+
+
+def vis_rotation(prop):
+    """rotate()'s rotation (beam_loop_mod.f90:1488-1503) taking prop to -z: (3, 3)."""
+    L = _vlib()
+    L.oracle_vis_rotation.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    p = _a(prop, np.float64)
+    R = np.empty((3, 3))
+    L.oracle_vis_rotation(p.ctypes.data, R.ctypes.data)
+    return R
+
+
+def vis_rotate(R, geom):
+    """The geometry fields the visibility routines read, rotated by R as rotate() does
+    (beam_loop_mod.f90:1505-1524): (v (nv,3), mid (nf,3), nz (nf), ap_nz (na))."""
+    L = _vlib()
+    vp = ctypes.c_void_p
+    L.oracle_vis_rotate.argtypes = [vp, _i64, vp, _i64, vp, vp, _i64, vp, vp, vp, vp, vp]
+    Rm = _a(R, np.float64)
+    v0 = _a(np.asarray(geom.v).T, np.float64)
+    mid0, n0, apn0 = _a(geom.mid, np.float64), _a(geom.n, np.float64), _a(geom.ap_n, np.float64)
+    nv, nf, na = v0.shape[1], len(mid0), len(apn0)
+    v, mid, nz, apz = np.empty((3, nv)), np.empty((nf, 3)), np.empty(nf), np.empty(na)
+    L.oracle_vis_rotate(Rm.ctypes.data, nv, v0.ctypes.data, nf, mid0.ctypes.data, n0.ctypes.data, na,
+                        apn0.ctypes.data, v.ctypes.data, mid.ctypes.data, nz.ctypes.data, apz.ctypes.data)
+    return v.T.copy(), mid, nz, apz

@@ -14,6 +14,7 @@
  *   find_vis_inc                  beam_loop_mod.f90:2461-2609
  *   beam_aligned_bounding_boxes   beam_loop_mod.f90:2668-2743
  *   compute_geometry_midpoints    misc_submod.f90:199-209 (for the boxes)
+ *   rotate                        beam_loop_mod.f90:1467-1526 (the beam frame)
  *
  * Inputs are the fields of geometry_type (types_mod.f90:156-190) that these
  * routines read, in the beam frame (propagation along -z):
@@ -36,6 +37,7 @@
  * construction against its closed form, and planted mistakes
  * (tests/test_vis_oracle.py).
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -256,6 +258,82 @@ void oracle_find_vis(int mode, int64_t nv, const double* v, int64_t nf, const in
     free(f5);
     free(is_beam);
     free(mapping);
+}
+
+/* rotate (beam_loop_mod.f90:1467-1526): the rotation taking the beam's propagation
+ * direction prop to -z, rot = rot2 . rot1, as R[3 i + k] = rot(i+1, k+1).  pi is the
+ * reference's REAL*4 literal (misc_submod.f90:12); cosine and sine of one angle are taken
+ * together (sincos), as gfortran does. */
+void oracle_vis_rotation(const double* prop, double* R)
+{
+    const double pi = 3.1415927410125732;
+    double r1[9] = {0}, r2[9] = {0}, c, s;
+    const double th1 = 2 * pi - atan2(prop[1], prop[0]);
+    sincos(th1, &s, &c);
+    r1[0] = c;
+    r1[1] = -s;
+    r1[3] = s;
+    r1[4] = c;
+    r1[8] = 1;
+    double t[3];
+    for (int i = 0; i < 3; i++) {          /* matmul(rot1, prop) */
+        double a = 0.0;
+        for (int k = 0; k < 3; k++) a = a + r1[3 * i + k] * prop[k];
+        t[i] = a;
+    }
+    const double nrm = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);   /* normalise_vec, misc_submod.f90:2216-2227 */
+    for (int i = 0; i < 3; i++) t[i] = t[i] / nrm;
+    const double th2 = acos(-t[2]);
+    sincos(th2, &s, &c);
+    r2[0] = c;
+    r2[2] = s;
+    r2[6] = -s;
+    r2[8] = c;
+    r2[4] = 1;
+    for (int i = 0; i < 3; i++)            /* matmul(rot2, rot1) */
+        for (int j = 0; j < 3; j++) {
+            double a = 0.0;
+            for (int k = 0; k < 3; k++) a = a + r2[3 * i + k] * r1[3 * k + j];
+            R[3 * i + j] = a;
+        }
+}
+
+/* matmul(rot, p) for one 3-vector, the reference's summation order */
+static void vis_rot3(const double* R, double px, double py, double pz, double* o)
+{
+    for (int i = 0; i < 3; i++) {
+        double a = 0.0;
+        a = a + R[3 * i] * px;
+        a = a + R[3 * i + 1] * py;
+        a = a + R[3 * i + 2] * pz;
+        o[i] = a;
+    }
+}
+
+/* The geometry fields the visibility routines read, rotated by R as rotate() does
+ * (vertices, facet normals, midpoints and aperture normals each rotated, nothing
+ * recomputed): v0 (nv,3) column-major, mid0 [nf][3], n0 [nf][3] (each facet's normal),
+ * apn0 [na][3] -> v (nv,3) column-major, mid [nf][3], nz [nf], ap_nz [na]. */
+void oracle_vis_rotate(const double* R, int64_t nv, const double* v0, int64_t nf, const double* mid0,
+                       const double* n0, int64_t na, const double* apn0, double* v, double* mid, double* nz,
+                       double* ap_nz)
+{
+    double o[3];
+    for (int64_t i = 0; i < nv; i++) {
+        vis_rot3(R, v0[i], v0[nv + i], v0[2 * nv + i], o);
+        v[i] = o[0];
+        v[nv + i] = o[1];
+        v[2 * nv + i] = o[2];
+    }
+    for (int64_t i = 0; i < nf; i++) {
+        vis_rot3(R, mid0[3 * i], mid0[3 * i + 1], mid0[3 * i + 2], mid + 3 * i);
+        vis_rot3(R, n0[3 * i], n0[3 * i + 1], n0[3 * i + 2], o);
+        nz[i] = o[2];
+    }
+    for (int64_t i = 0; i < na; i++) {
+        vis_rot3(R, apn0[3 * i], apn0[3 * i + 1], apn0[3 * i + 2], o);
+        ap_nz[i] = o[2];
+    }
 }
 
 /* Many beams over one facet topology, OpenMP over beams (the CPU baseline).  Per beam b:

@@ -238,3 +238,37 @@ def test_beams_past_one_launch_and_empty_beams():
     _same(r, ref, f"{nb} beams")
     np.testing.assert_array_equal(cnt, rc.sum(0))
     assert not r["in_beam"][::997].any()
+
+
+def test_rotate_dev_and_end_to_end():
+    """The beam frames on the GPU (abt_vis_rotate_dev) bit-identical to rotate(), and the whole
+    chain (rotations, rotated geometry, batched find_vis_int) against the oracle's."""
+    import torch
+    v, f, a, n = hex_column_apertures(10.0, 20.0, 8, 14, sigma=0.5, seed=0)
+    g0 = V.VisGeometry.from_mesh(v, f, a, n)
+    rng = np.random.default_rng(12)
+    props = rng.normal(size=(64, 3))
+    props /= np.linalg.norm(props, axis=1, keepdims=True)
+    R = V.vis_rotations(props)
+    dev = torch.device("cuda:0")
+    print("vis rotate_dev 64 beams", flush=True)
+    rv, rmid, rnz, rapz = V.rotate_dev(torch.from_numpy(R).to(dev), g0)
+    torch.cuda.synchronize()
+    rv, rmid, rnz, rapz = (x.cpu().numpy() for x in (rv, rmid, rnz, rapz))
+    fins, ino = [], [0]
+    for b in range(64):
+        ov, omid, onz, oapz = O.vis_rotate(R[b], g0)
+        np.testing.assert_array_equal(rv[b], ov.T)
+        np.testing.assert_array_equal(rmid[b], omid)
+        np.testing.assert_array_equal(rnz[b], onz)
+        np.testing.assert_array_equal(rapz[b], oapz)
+        up = np.nonzero(oapz > 0.01)[0]
+        fi = np.nonzero(g0.ap == up[b % len(up)] + 1)[0] + 1
+        fins.append(fi)
+        ino.append(ino[-1] + len(fi))
+    args = (g0, rv, rmid, rnz, rapz, np.array(ino, np.int64), np.concatenate(fins), np.zeros(64))
+    print("vis end-to-end int 64 beams", flush=True)
+    r, cnt = _batch("int", *args)
+    ref, rc = O.find_vis_batch("int", *args, nthreads=16)
+    _same(r, ref, "end-to-end")
+    assert r["in_beam"].sum() > 0

@@ -107,3 +107,13 @@ def test_vis_workload_shapes():
     for b in range(3):   # the illuminating facets are one down-facing aperture
         a = np.unique(topo.ap[fin[ino[b]:ino[b + 1]] - 1])
         assert len(a) == 1 and apn[b, a[0] - 1] < -0.01
+
+
+def test_host_rotations_match_oracle_bitwise(orc):
+    """abt_vis_rotations (host, C library) against the oracle's restatement of rotate()."""
+    rng = np.random.default_rng(9)
+    p = rng.normal(size=(50, 3))
+    p /= np.linalg.norm(p, axis=1, keepdims=True)
+    R = V.vis_rotations(p)
+    for b in range(len(p)):
+        np.testing.assert_array_equal(R[b], orc.vis_rotation(p[b]))

@@ -268,3 +268,48 @@ def test_planted_mistakes_are_caught(tmp_path, monkeypatch, old, new, which):
         ok = (np.array_equal(r["in_beam"], lit) and np.array_equal(r["is_vis"], tested & ~blocked)
               and np.array_equal(r["id_beam"][lit], idb[lit]))
     assert not ok
+
+
+# ---- rotate(): the beam frame ----
+
+PI_R4 = 3.1415927410125732
+
+
+def _check_rotation(orc, prop):
+    """rot takes prop to -z up to the reference's REAL*4 pi: the first rotation about z turns
+    prop's azimuth to 2 pi_R4 instead of 2 pi, leaving a y part |p_xy| sin(2 (pi_R4 - pi))."""
+    R = orc.vis_rotation(prop)
+    p = np.asarray(prop) / np.linalg.norm(prop)
+    q = R @ p
+    ok = np.allclose(R @ R.T, np.eye(3), atol=1e-15) and abs(np.linalg.det(R) - 1) < 1e-15
+    ok &= abs(q[0]) < 1e-13 and abs(q[2] + 1) < 1e-13
+    ok &= abs(q[1] - np.hypot(p[0], p[1]) * np.sin(2 * (PI_R4 - np.pi))) < 4e-15
+    return ok
+
+
+def test_beam_rotation_closed_form():
+    rng = np.random.default_rng(3)
+    props = list(rng.normal(size=(40, 3)))
+    props = [p / np.linalg.norm(p) for p in props] + [np.array([1.0, 0, 0]), np.array([0, -1.0, 0]),
+                                                       np.array([0.6, 0.0, -0.8])]
+    for p in props:
+        assert _check_rotation(O, p), p
+
+
+def test_beam_rotation_planted_true_pi(tmp_path, monkeypatch):
+    L = _planted(tmp_path, "const double pi = 3.1415927410125732;", "const double pi = 3.141592653589793;")
+    monkeypatch.setattr(O, "_lib", L)
+    monkeypatch.setattr(O, "_vis_ready", True)
+    assert not _check_rotation(O, np.array([0.48, 0.6, -0.64]))
+
+
+def test_rotated_geometry_is_the_rotation():
+    """rotate() rotates vertices, midpoints and normals (nothing recomputed): against numpy."""
+    v, f, a, n = hex_column_apertures(3.0, 5.0, 2, 3, sigma=0.2, seed=1)
+    g = VisGeometry.from_mesh(v, f, a, n)
+    R = O.vis_rotation(np.array([0.3, -0.5, 0.81]) / np.linalg.norm([0.3, -0.5, 0.81]))
+    rv, rmid, rnz, rapz = O.vis_rotate(R, g)
+    np.testing.assert_allclose(rv, g.v @ R.T, atol=1e-14)
+    np.testing.assert_allclose(rmid, g.mid @ R.T, atol=1e-14)
+    np.testing.assert_allclose(rnz, g.n @ R[2], atol=1e-15)
+    np.testing.assert_allclose(rapz, g.ap_n @ R[2], atol=1e-15)
