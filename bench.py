@@ -343,7 +343,11 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
     s11 = np.abs(M64[0])
     norm = np.abs(mm - M64) / s11
     s11r = np.abs(mm[0] - M64[0]) / s11
+    # the acceptance metric |dS_ij| / max(1e-3 |S11|, |S_ij|) (floor 1e-3 S11), per element
+    flo = np.abs(mm - M64) / np.maximum(1e-3 * s11, np.abs(M64))
     q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
+    bad_e = np.nonzero((flo[1:] > 1e-5).any(0))[0]
+    bad_s = np.nonzero(s11r > 1e-6)[0]
     asym_m, asym_64 = Im[:, 1] / Im[:, 0], I64h[:, 1] / I64h[:, 0]
     res.update({
         "fp64_plain": {"value": v_plain, "ms": dt_plain * 1e3,
@@ -353,6 +357,10 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
         "accuracy": {"reference": "FP64 batched kernel on the same prepared beams",
                      "S11_rel": q(s11r), "norm_elem": q(norm),
                      "per_element_norm_max": [float(norm[e].max()) for e in range(16)],
+                     "rel_floored": {"floor": "1e-3 S11", "per_element_max": [float(flo[e].max()) for e in range(16)],
+                                     "per_element_p999": [float(np.quantile(flo[e], 0.999)) for e in range(16)]},
+                     "flagged_bins": {"elements_gt_1e-5": int(len(bad_e)), "S11_gt_1e-6": int(len(bad_s)),
+                                      "first": [int(x) for x in np.union1d(bad_e, bad_s)[:20]]},
                      "plain_vs_batched_fp64_norm_max": float((np.abs(Mp - M64) / s11).max()),
                      "asymmetry_abs_max": float(np.abs(asym_m - asym_64).max()),
                      "scatt_rel_max": float(np.abs(Im[:, 0] / I64h[:, 0] - 1).max())},
