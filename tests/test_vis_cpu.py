@@ -117,3 +117,13 @@ def test_host_rotations_match_oracle_bitwise(orc):
     R = V.vis_rotations(p)
     for b in range(len(p)):
         np.testing.assert_array_equal(R[b], orc.vis_rotation(p[b]))
+
+
+def test_rotation_entry_points_reject_bad_input():
+    L = V._lib()
+    assert L.abt_vis_rotations(-1, None, None) == 1
+    assert L.abt_vis_rotations(2, None, None) == 1
+    assert L.abt_vis_rotate_dev(1, None, 4, None, 4, None, None, 1, None, None, None, None, None, None) == 1
+    assert "null array" in L.abt_last_error().decode()
+    assert L.abt_vis_rotate_dev(1, None, 0, None, 4, None, None, 1, None, None, None, None, None, None) == 1
+    assert "sizes" in L.abt_last_error().decode()
