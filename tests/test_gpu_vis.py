@@ -272,3 +272,16 @@ def test_rotate_dev_and_end_to_end():
     ref, rc = O.find_vis_batch("int", *args, nthreads=16)
     _same(r, ref, "end-to-end")
     assert r["in_beam"].sum() > 0
+
+
+def test_host_api_is_reentrant():
+    """Several host threads calling abt_find_vis at once (the tracer's OpenMP threads in
+    -memory mode): each gets the oracle's result."""
+    import concurrent.futures as cf
+    topo, v, mid, nz, apn, ino, fin, bmz = vis_workload(16, "int", seed=13)
+    geos = [(_geom(topo, v, mid, nz, apn, b), fin[ino[b]:ino[b + 1]]) for b in range(16)]
+    print("vis host api from 4 threads", flush=True)
+    with cf.ThreadPoolExecutor(4) as ex:
+        got = list(ex.map(lambda gf: V.find_vis("int", gf[0], gf[1]), geos))
+    for b, (g, f) in enumerate(geos):
+        _same(got[b], O.find_vis("int", g, f), f"thread beam {b}")
