@@ -41,12 +41,13 @@ namespace abt {
 constexpr int BLOCK = 256;
 
 // ---------------------------------------------------------------------------
+// The mixed kernel with up to 16-beam tiles is held to 128 VGPRs (4 waves per SIMD), its
+// fastest occupancy on gfx950.
 template <int PREC, int TB>
-__global__ __launch_bounds__(BLOCK) void k_far_field(DevGrid g, const double* __restrict__ recs,
-                                                     const double* __restrict__ vrecs,
-                                                     int64_t nb, int64_t per_chunk, int fast_diff,
-                                                     double* __restrict__ out,
-                                                     double* __restrict__ work)
+__global__ __launch_bounds__(BLOCK, (PREC == 1 && TB <= 16) ? 4 : 1)
+void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __restrict__ vrecs,
+                 int64_t nb, int64_t per_chunk, int fast_diff, double* __restrict__ out,
+                 double* __restrict__ work)
 {
     // Mixed kernel: two record tiles; the next tile's records are loaded into registers
     // while the current one is summed, then stored to the other buffer (one barrier per

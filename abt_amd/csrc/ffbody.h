@@ -455,8 +455,29 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
         const float sq = s2c * rs;                                 // sqrt(1 - k2^2)
         const float frac = sq * rf[F_SIG];
         const float ifrac = rs * rf[F_ISIG];
-        const float da = -0.5f * fmaf(rf[F_KZ], frac, fk3 * ifrac);          // diff_ampl(1,1) = (2,2)
-        const float db = 0.5f * fmaf(rf[F_KXY], frac, -fk1 * fk2c * ifrac);  // diff_ampl(1,2) = -(2,1)
+        float da = -0.5f * fmaf(rf[F_KZ], frac, fk3 * ifrac);          // diff_ampl(1,1) = (2,2)
+        float db = 0.5f * fmaf(rf[F_KXY], frac, -fk1 * fk2c * ifrac);  // diff_ampl(1,2) = -(2,1)
+        {
+            // diff_ampl vanishes where k is the mirror image of prop2 in the aperture plane,
+            // (k_x, k_y, k_z) = (K_x, K_y, -K_z): for every externally reflected beam that is
+            // the forward direction, so thousands of beams share the zero there.  Through
+            // dk = (k_x - K_x, k_y - K_y, k_z + K_z) (the first two from q, the third in FP64)
+            // and u2 = k_y^2 - K_y^2 = dk_y (k_y + K_y), w2 = K_x dk_y + dk_x k_y:
+            //   K_z (1 - k_y^2) + k_z (1 - K_y^2) = -K_z u2 + (1 - K_y^2) dk_z,
+            //   K_x K_y (1 - k_y^2) - k_x k_y (1 - K_y^2) = -K_x K_y u2 - (1 - K_y^2) w2,
+            // whose terms vanish with dk.  Used where |k_y| < 1/sqrt(2) (aperture frames put
+            // prop2 in the x-z plane, K_y = 0, so that covers the zero); nearer k_y = +-1 the
+            // terms of u2 outgrow 1 - k_y^2 and the direct form is kept.
+            const float fdk1 = -fqx * ik, fdk2 = -fqy * ik;
+            const float fdk3 = (float)fma(RR[6], ux, fma(RR[7], uy, fma(RR[8], uz, r[R_PROP2 + 2])));
+            const float u2 = fdk2 * (fk2 + rf[F_KY]);
+            const float w2 = fmaf(rf[F_KX], fdk2, fdk1 * fk2);
+            const float dan = 0.5f * rs * fmaf(rf[F_KZ] * rf[F_SIG], u2, -rf[F_ISIG] * fdk3);
+            const float dbn = -0.5f * rs * fmaf(rf[F_KXY] * rf[F_SIG], u2, rf[F_ISIG] * w2);
+            const bool mir = !kclamp && s2 > 0.5f;
+            da = mir ? dan : da;
+            db = mir ? dbn : db;
+        }
         const float m1 = -fk1 * fk2c * rs, m2 = sq, m3 = -fk2c * fk3 * rs;
         const float r13 = kclamp ? abt_rsqf(s2) : rs;
         const float ev1 = -fk3 * r13, ev3 = fk1 * r13;

@@ -86,6 +86,29 @@ def random_triangles(n, seed=0, box=10.0, size=(0.5, 4.0), amp_decades=0.0) -> B
     return BeamSet.from_triangles(a, vk7, prop, np.full(n, np.pi / 2), tri)
 
 
+def reflected_facets(n, seed=0, box=40.0, size=(1.0, 4.0)) -> BeamSet:
+    """External reflections of the incident wave (0,0,-1): n triangular facets with seeded
+    normals (n_z >= 0.2, lit from above), each aperture the facet itself and its beam the
+    mirror image prop = inc - 2 (inc.n) n, e-perp unit(n x prop).  The forward direction
+    (0,0,-1) is the mirror image of every beam's prop in its aperture plane, where the
+    Karczewski-Wolf amplitude of every beam vanishes."""
+    rng = np.random.default_rng(seed)
+    nrm = _unit(rng.normal(size=(n, 3)))
+    nrm[:, 2] = np.abs(nrm[:, 2])
+    nrm = _unit(np.where(nrm[:, 2:3] < 0.2, nrm + [0.0, 0.0, 0.4], nrm))
+    inc = np.array([0.0, 0.0, -1.0])
+    prop = inc - 2 * (nrm @ inc)[:, None] * nrm
+    vk7 = _unit(np.cross(nrm, prop))
+    a, b = _perp_basis(nrm, rng)
+    c = rng.uniform(-box / 2, box / 2, (n, 3))
+    ang = rng.uniform(0, 2 * np.pi, (n, 1))
+    r = rng.uniform(*size, (n, 1))
+    tri = np.stack([c + r * (np.cos(ang + t) * a + np.sin(ang + t) * b)
+                    for t in (0.0, 2.1, 4.2)], 1)
+    amp = rng.normal(size=(n, 8))
+    return BeamSet.from_triangles(amp, vk7, prop, np.full(n, np.pi / 2), tri)
+
+
 def stress_beams(n=200_000, seed=20240601) -> BeamSet:
     """Diffraction stress test: n planar triangles, centroids uniform in a 100 um cube,
     two edge vectors of length U(0.3, 5) um in the plane normal to the propagation
