@@ -52,6 +52,8 @@ def main():
                     help="run the FP64 plain and FP64 batched comparisons of the orientation average (0: skip)")
     ap.add_argument("--adjoint-orient", type=int, default=64,
                     help="orientations of the adjoint workload (0: skip)")
+    ap.add_argument("--large-orient", type=int, default=0,
+                    help="orientations of the large absorbing column (20,736 facets; 0: skip)")
     ap.add_argument("--vis-beams", type=int, default=4096,
                     help="beams of the facet-visibility workload (0: skip)")
     ap.add_argument("--batch-tiles", default=None,
@@ -168,6 +170,9 @@ def main():
                                                          args.batch_tiles)
     if args.adjoint_orient > 0:
         out["adjoint"] = adjoint_bench(args.adjoint_orient, lam, dev)
+    if args.large_orient > 0:
+        out["large_column"] = orientation_average(args.large_orient, args.steps, lam, dev, True, None,
+                                                  particle="large_column", grid_name="large", with_kappa=True)
     if args.vis_beams > 0:
         out["visibility"] = visibility_bench(args.vis_beams, max(args.steps, 20), max(args.warmup, 2), dev,
                                              args.cpu_sample > 0)
@@ -228,7 +233,8 @@ def _prepare_trees(w, lam, dev):
     return rb, re, nb, ne
 
 
-def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=None):
+def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=None, particle="rough_column",
+                        grid_name="orient", with_kappa=False):
     """Batched orientation averaging: `norient` orientations (uniform on SO(3), seeded) of
     a Gaussian-rough hexagonal column (radius 10 um, length 20 um, sigma 0.5 um, 2112
     facets), 10 internal reflections, 721 theta x 180 phi bins.  The whole job (every
@@ -236,16 +242,19 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
     incoherently, phi integral, per-orientation integrals) is one graph-captured
     persistent launch with the autotuned tile shape, in mixed precision.  Beside it: the
     plain FP64 drop-in path run as the reference drivers do (per orientation diff_main,
-    finalise, summation), and the FP64 batched kernel as the accuracy reference."""
+    finalise, summation), and the FP64 batched kernel as the accuracy reference.  With
+    particle="large_column", grid_name="large": the absorbing 20,736-facet column (radius
+    50 um, length 100 um, m 1.31+0.01i, 12 reflections) on 451 theta (0.05 deg to 5 deg,
+    then 0.5 deg) x 360 phi bins."""
     import torch
     from abt_amd import lib
     from abt_amd.workloads import GRIDS, grid, orientation_workload
     t0 = time.time()
-    w = orientation_workload(norient, device=dev)
+    w = orientation_workload(norient, particle, device=dev)
     torch.cuda.synchronize()
     boff, eoff = w["beam_off"].cpu().numpy(), w["ext_off"].cpu().numpy()
     counts = np.diff(boff) + np.diff(eoff)
-    th, ph = grid(*GRIDS["orient"])
+    th, ph = grid(*GRIDS[grid_name])
     g = lib.DeviceGrid(th, ph, lam)
     rb, re, nb, ne = _prepare_trees(w, lam, dev)
     del w
@@ -288,9 +297,12 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
     log(f"  mixed batched (graph): {ms:.1f} ms/step, {v_mixed:.4e} beam.bin/s, tiles {tiles}")
     mm, Im = m.cpu().numpy().reshape(16, -1), I.cpu().numpy().reshape(-1, 6)
     res = {
-        "workload": f"{norient} orientations (uniform SO(3), seed 1) of a rough hexagonal column "
-                    "(r 10 um, l 20 um, sigma 0.5 um, 2112 facets), m 1.31, 10 reflections, "
-                    "721 theta x 180 phi bins, lambda 0.532 um",
+        "workload": (f"{norient} orientations (uniform SO(3), seed 1) of a rough hexagonal column "
+                     "(r 10 um, l 20 um, sigma 0.5 um, 2112 facets), m 1.31, 10 reflections, "
+                     "721 theta x 180 phi bins, lambda 0.532 um") if particle == "rough_column" else
+                    (f"{norient} orientations (uniform SO(3), seed 1) of the absorbing hexagonal column "
+                     "(r 50 um, l 100 um, 20736 facets), m 1.31+0.01i, 12 reflections, "
+                     f"{len(th)} theta x {len(ph)} phi bins, lambda 0.532 um"),
         "beams": int(nb), "ext_beams": int(ne), "beams_per_orientation": [int(counts.min()), int(counts.max())],
         "beam_bins": bb, "metric": "beam·bin diffractions/s", "value": v_mixed, "ms_per_step": ms,
         "steps": steps, "launch": "one hipGraph replay per step (persistent kernel + reductions)",
@@ -340,6 +352,20 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
     v_plain = bb / dt_plain
     log(f"  fp64 plain loop: {dt_plain * 1e3:.1f} ms, {v_plain:.4e} beam.bin/s")
     Mp = tot.cpu().numpy().reshape(16, -1)
+    kappa = None
+    if with_kappa:
+        # conditioning of the coherent sums: kappa^2 = sum_o sum_b |A_b|^2 / sum_o |sum_b A_b|^2 (FP64),
+        # with sum_o |X_o|_F^2 = 2 M11 for X = k A (the Mueller matrix's 1/2 convention)
+        incs = torch.zeros(nbins, **f64)
+        tmp = torch.empty(nbins, **f64)
+        for o in range(norient):
+            for rr, oo in ((rb, boff), (re, eoff)):
+                a0, a1 = int(oo[o]), int(oo[o + 1])
+                if a1 > a0:
+                    lib.far_field_incoherent_dev(g, rr[a0:a1], a1 - a0, tmp)
+                    incs.add_(tmp)
+        torch.cuda.synchronize()
+        kappa = np.sqrt(incs.cpu().numpy() * (2 * lib.PI_R4 / lam) ** 2 / (2 * np.abs(M64[0])))
     s11 = np.abs(M64[0])
     norm = np.abs(mm - M64) / s11
     s11r = np.abs(mm[0] - M64[0]) / s11
@@ -361,6 +387,7 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
                                      "per_element_p999": [float(np.quantile(flo[e], 0.999)) for e in range(16)]},
                      "flagged_bins": {"elements_gt_1e-5": int(len(bad_e)), "S11_gt_1e-6": int(len(bad_s)),
                                       "first": [int(x) for x in np.union1d(bad_e, bad_s)[:20]]},
+                     **({"kappa": _kappa_report(kappa, flo, s11r, s11, th, ph)} if kappa is not None else {}),
                      "plain_vs_batched_fp64_norm_max": float((np.abs(Mp - M64) / s11).max()),
                      "asymmetry_abs_max": float(np.abs(asym_m - asym_64).max()),
                      "scatt_rel_max": float(np.abs(Im[:, 0] / I64h[:, 0] - 1).max())},
@@ -368,6 +395,36 @@ def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=No
     for x in (plan, plan64):
         x.close()
     return res
+
+
+def _kappa_report(kappa, flo, s11r, s11, th, ph):
+    """Errors against the conditioning kappa of each bin's coherent sums (a per-beam relative
+    error eps shows up as ~eps kappa): quantiles of kappa, the share of flagged bins that are
+    ill-conditioned (kappa > 10), the errors divided by max(kappa, 1) (the per-beam precision;
+    kappa < 1 where the beams add coherently), the flagged bins by the depth of S11 below its
+    largest value, and where the largest errors sit."""
+    q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
+    smax = s11.max()
+    where = lambda i: {"theta_deg": float(np.degrees(th[i // len(ph)])), "phi_deg": float(np.degrees(ph[i % len(ph)])),
+                       "kappa": float(kappa[i]), "S11_rel": float(s11r[i]), "S11_over_max": float(s11[i] / smax)}
+    worst = flo[1:].max(0)
+    flag = (worst > 1e-5) | (s11r > 1e-6)
+    well = kappa <= 10
+    return {"median": float(np.median(kappa)), "p99": float(np.quantile(kappa, 0.99)), "max": float(kappa.max()),
+            "flagged_with_kappa_gt_10": int((flag & ~well).sum()), "flagged_with_kappa_le_10": int((flag & well).sum()),
+            "well_conditioned_bins": int(well.sum()),
+            "S11_rel_kappa_le_10": q(s11r[well]) if well.any() else None,
+            "floored_kappa_le_10": q(worst[well]) if well.any() else None,
+            "S11_rel_over_kappa": q(s11r / np.maximum(kappa, 1.0)),
+            "floored_over_kappa": q(worst / np.maximum(kappa, 1.0)),
+            "flagged_by_S11_level": {"S11_ge_1e-4_max": int((flag & (s11 >= 1e-4 * smax)).sum()),
+                                     "S11_1e-6_to_1e-4_max": int((flag & (s11 < 1e-4 * smax) & (s11 >= 1e-6 * smax)).sum()),
+                                     "S11_lt_1e-6_max": int((flag & (s11 < 1e-6 * smax)).sum())},
+            "bins_by_S11_level": {"S11_ge_1e-4_max": int((s11 >= 1e-4 * smax).sum()),
+                                  "S11_1e-6_to_1e-4_max": int(((s11 < 1e-4 * smax) & (s11 >= 1e-6 * smax)).sum()),
+                                  "S11_lt_1e-6_max": int((s11 < 1e-6 * smax).sum())},
+            "worst_S11_bins": [where(i) for i in np.argsort(-s11r)[:5]],
+            "worst_S11_over_kappa_bins": [where(i) for i in np.argsort(-(s11r / np.maximum(kappa, 1.0)))[:5]]}
 
 
 def adjoint_bench(norient, lam, dev):
