@@ -5,9 +5,9 @@
 // Beam records are staged in LDS one tile at a time (cooperative coalesced
 // loads, then broadcast reads) and every lane walks the tile in beam order, so
 // each bin's sum is formed in a fixed order and is run-to-run deterministic.
-// blockIdx.y splits the beam list into chunks when there are too few bin tiles
-// to fill the 256 CUs; chunk partials are then summed in chunk order by
-// k_reduce_chunks.
+// blockIdx.y splits the beam list into chunks so that a launch has many short
+// workgroups (a small last round); chunk partials are then summed in chunk order
+// by k_reduce_chunks.
 //
 // k_ff_fp64  -- the reference's formulas in FP64, operation for operation
 //               (diffraction, diff_mod.f90:576-670; karczewski :124-198;
