@@ -140,7 +140,8 @@ int abt_grid_create(const abt_grid* grid, double lambda, abt_dev_grid** out);
 int abt_grid_destroy(abt_dev_grid* g);
 int64_t abt_grid_bins(const abt_dev_grid* g);
 
-/* Workspace bytes abt_far_field_dev needs for nb beams (0 when none). */
+/* Workspace bytes abt_far_field_dev needs for nb beams (0 when none): 64 bytes per bin
+ * per beam chunk, at most 32 chunks of >= 512 beams (2048 bytes per bin). */
 size_t abt_far_field_workspace(const abt_dev_grid* g, int64_t nb, int precision);
 
 /* Coherent far-field sum of nb prepared beams: dev_out (complex, Fortran layout,
