@@ -201,7 +201,8 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
                           const int64_t* ext_off, int precision, int is_fast_diff,
                           abt_batch_plan** out);
 int abt_batch_plan_destroy(abt_batch_plan* plan);
-/* Device bytes the plan holds (partial sums, weights, tables). */
+/* Device bytes the plan holds (partial sums: 128 bytes per bin per orientation chunk, up to
+ * 128 chunks; weights, tables). */
 int abt_batch_workspace(const abt_batch_plan* plan, size_t* bytes);
 /* Tile shape: beams staged per LDS tile (8, 16, 32) and bins per lane (1, 2).
  * Default 16 x 1 until set or autotuned (ABT_FP64_ACCURATE: 16 x 1 only). */
