@@ -41,11 +41,11 @@ namespace abt {
 
 constexpr int BB = 256;   // lanes per workgroup
 
-template <int PREC, int TB, int BPT>
+template <int PREC, int TB, int BPT, bool INC>
 __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restrict__ recs,
                                       const double* __restrict__ vrecs, int64_t b0, int64_t b1,
-                                      double* tile, const BinIn* bi, const double* cp,
-                                      const double* sp, double (*acc)[8])
+                                      double* tile, const double* etab, const BinIn* bi, const double* cp,
+                                      const double* sp, double (*acc)[8], double* inc)
 {
     const int tid = threadIdx.x;
     for (int64_t t0 = b0; t0 < b1; t0 += TB) {
@@ -58,21 +58,25 @@ __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restri
             // FP32 partial sums over groups of MIXED_GROUP beams (counted from the
             // orientation's first beam; TB is a multiple), added into FP64
             for (int g0 = 0; g0 < nt; g0 += MIXED_GROUP) {
-                float accf[BPT][8];
+                float accf[BPT][8], incf[BPT];
 #pragma unroll
-                for (int j = 0; j < BPT; j++)
+                for (int j = 0; j < BPT; j++) {
+                    incf[j] = 0.f;
 #pragma unroll
                     for (int q = 0; q < 8; q++) accf[j][q] = 0.f;
+                }
                 const int g1 = min(nt, g0 + MIXED_GROUP);
                 for (int bb = g0; bb < g1; bb++) {
 #pragma unroll
                     for (int j = 0; j < BPT; j++)
-                        beam_bin_mixed(tile + bb * REC, vrecs, a.g, bi[j], a.fast_diff, accf[j]);
+                        beam_bin_mixed_t<INC>(tile + bb * REC, vrecs, a.g, bi[j], a.fast_diff, accf[j], incf[j]);
                 }
 #pragma unroll
-                for (int j = 0; j < BPT; j++)
+                for (int j = 0; j < BPT; j++) {
+                    if (INC) inc[j] += (double)incf[j];
 #pragma unroll
                     for (int q = 0; q < 8; q++) acc[j][q] += (double)accf[j][q];
+                }
             }
         } else {
             for (int bb = 0; bb < nt; bb++) {
@@ -80,6 +84,8 @@ __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restri
                 for (int j = 0; j < BPT; j++) {
                     if (PREC == 0)
                         beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
+                    else if (PREC == 4)
+                        beam_bin_fp64r(tile + bb * REC, vrecs, a.g, bi[j], etab, a.fast_diff, acc[j]);
                     else {
                         AdjBeam B;
                         adj_beam_from_record(tile + bb * REC, B);
@@ -103,43 +109,68 @@ __device__ __forceinline__ double wave_sum(double v)
 // the ext tree is summed, and the phi rotation and integral weights are re-read
 // after the sweeps, so the live state across a sweep is the accumulators and the
 // bin position only.
-template <int PREC, int TB, int BPT>
+//
+// MODE == BMODE_EST (mixed only): also the error scale of the FP32 body, the sum over
+// orientations of S11_o * I_o with I_o = k^2 sum_b |A_b|_F^2 the bin's incoherent beam sum,
+// into epart.  MODE == BMODE_LIST: lane slots are positions in the list of bins a.list
+// (*a.list_count of them), whose Mueller sums go to mpart at the list position; no
+// integrals (the mixed pass keeps them).
+template <int PREC, int TB, int BPT, int MODE>
 __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
 {
+    constexpr bool INC = MODE == BMODE_EST;
+    constexpr bool LIST = MODE == BMODE_LIST;
+    constexpr bool PROJ = PREC == 1 || PREC == 4;   // projector sums, finished by mixed_finish
     __shared__ double tile[TB * REC];
+    __shared__ double etab[PREC == 4 ? ETAB_DOUBLES : 1];   // exponential table of the FP64r body
     __shared__ double xbs[8][BB * BPT];
     __shared__ double red[BB / 64][6];
     __shared__ unsigned s_unit;
     const int tid = threadIdx.x;
     const int nbins = a.g.nbins;
     const double k = a.g.waveno;
+    if (PREC == 4)   // published by the barrier at the top of the unit loop
+        for (int q = tid; q < ETAB_DOUBLES; q += BB) etab[q] = a.g.etab[q];
+    int ntiles = a.ntiles, nunits = a.nunits, nslots = nbins;
+    if (LIST) {
+        nslots = *a.list_count;
+        ntiles = (nslots + BB * BPT - 1) / (BB * BPT);
+        nunits = ntiles * a.nchunks;
+    }
     for (;;) {
         __syncthreads();
         if (tid == 0) s_unit = atomicAdd(a.counter, 1u);
         __syncthreads();
         const unsigned u = s_unit;
-        if (u >= (unsigned)a.nunits) break;   // uniform over the workgroup
-        const int c = (int)(u / (unsigned)a.ntiles);
-        const int t = (int)(u - (unsigned)c * (unsigned)a.ntiles);
-        const int bin0 = t * BPT * BB + tid;   // bin of slot j: bin0 + j * BB
+        if (u >= (unsigned)nunits) break;   // uniform over the workgroup
+        const int c = (int)(u / (unsigned)ntiles);
+        const int t = (int)(u - (unsigned)c * (unsigned)ntiles);
+        const int slot0 = t * BPT * BB + tid;   // slot of lane j: slot0 + j * BB
 
         BinIn bi[BPT];
         double cp[BPT], sp[BPT];
+        int bins[BPT];
 #pragma unroll
         for (int j = 0; j < BPT; j++) {
-            const int bc = min(bin0 + j * BB, nbins - 1);
+            const int sc = min(slot0 + j * BB, nslots - 1);
+            const int bc = LIST ? a.list[sc] : sc;
+            bins[j] = bc;
             const int iphi = bc % a.g.nphi;
             bi[j].x = a.g.x[bc]; bi[j].y = a.g.y[bc]; bi[j].z = a.g.z[bc];
             const int bt = iphi + a.g.nphi * a.g.theta_i;
             bi[j].xt = a.g.x[bt]; bi[j].yt = a.g.y[bt]; bi[j].zt = a.g.z[bt];
             bi[j].rad = PREC == 3 ? a.g.rad[bc] : 0.0;
-            cp[j] = PREC != 1 ? a.g.cphi[iphi] : 0.0;   // the mixed body rotates after the sum
-            sp[j] = PREC != 1 ? a.g.sphi[iphi] : 0.0;
+            cp[j] = PROJ ? 0.0 : a.g.cphi[iphi];   // the projector bodies rotate after the sum
+            sp[j] = PROJ ? 0.0 : a.g.sphi[iphi];
         }
         const int o0 = a.chunk_o[c], o1 = a.chunk_o[c + 1];
         double* mp = a.mpart + (int64_t)c * 16 * nbins;
+        double* ep = INC ? a.epart + (int64_t)c * nbins : nullptr;
         for (int o = o0; o < o1; o++) {
             double acc[BPT][8];
+            double inc[BPT];
+#pragma unroll
+            for (int j = 0; j < BPT; j++) inc[j] = 0.0;
             // segment 0: the beam tree (result parked in LDS), segment 1: the ext tree
 #pragma nounroll
             for (int seg = 0; seg < 2; seg++) {
@@ -150,18 +181,15 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                 const double* recs = seg ? a.recs_e : a.recs_b;
                 const double* vrecs = seg ? a.vrecs_e : a.vrecs_b;
                 const int64_t* off = seg ? a.eoff : a.boff;
-                sweep<PREC, TB, BPT>(a, recs, vrecs, off[o], off[o + 1], tile, bi, cp, sp, acc);
+                sweep<PREC, TB, BPT, INC>(a, recs, vrecs, off[o], off[o + 1], tile, etab, bi, cp, sp, acc, inc);
                 if (seg == 1) break;
 #pragma unroll
                 for (int j = 0; j < BPT; j++) {
-                    if (PREC == 1) {
-                        const int iphi = min(bin0 + j * BB, nbins - 1) % a.g.nphi;
+                    if (PROJ) {
+                        const int iphi = bins[j] % a.g.nphi;
                         mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                     }
-                    if (PREC == 3) {
-                        const int bc = min(bin0 + j * BB, nbins - 1);
-                        model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bc]);
-                    }
+                    if (PREC == 3) model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bins[j]]);
 #pragma unroll
                     for (int q = 0; q < 8; q++)
                         xbs[q][j * BB + tid] = acc[j][q] * k;   // ampl_far_beam * waveno (outputs_mod.f90:104)
@@ -170,9 +198,9 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
             double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < BPT; j++) {
-                const int bin = bin0 + j * BB;
-                const int bc = min(bin, nbins - 1);
-                if (PREC == 1) {
+                const int slot = slot0 + j * BB;
+                const int bc = bins[j];
+                if (PROJ) {
                     const int iphi = bc % a.g.nphi;
                     mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                 }
@@ -185,13 +213,13 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                     xt[q] = xb[q] + xe[q];      // outputs_mod.f90:108
                 }
                 mueller16(xt, m);
-                if (bin < nbins) {
+                if (slot < nslots) {
                     if (a.ampl_out) {
-                        double* ao = a.ampl_out + (int64_t)o * 8 * nbins + bin;
+                        double* ao = a.ampl_out + (int64_t)o * 8 * nbins + bc;
 #pragma unroll
                         for (int q = 0; q < 8; q++) ao[(int64_t)q * nbins] = xt[q];
                     }
-                    double* mo = mp + bin;
+                    double* mo = mp + slot;
                     if (o == o0) {
 #pragma unroll
                         for (int e = 0; e < 16; e++) mo[(int64_t)e * nbins] = m[e];
@@ -199,29 +227,37 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
 #pragma unroll
                         for (int e = 0; e < 16; e++) mo[(int64_t)e * nbins] += m[e];
                     }
-                    const double w1 = a.w1[bin], w2 = a.w2[bin];
-                    const double sb = mueller_s11(xb), se = mueller_s11(xe);
-                    v[0] = fma(w1, m[0], v[0]);
-                    v[1] = fma(w2, m[0], v[1]);
-                    v[2] = fma(w1, sb, v[2]);
-                    v[3] = fma(w2, sb, v[3]);
-                    v[4] = fma(w1, se, v[4]);
-                    v[5] = fma(w2, se, v[5]);
+                    if (INC) {
+                        const double es = m[0] * (inc[j] * (k * k));
+                        if (o == o0) ep[slot] = es; else ep[slot] += es;
+                    }
+                    if (!LIST) {
+                        const double w1 = a.w1[bc], w2 = a.w2[bc];
+                        const double sb = mueller_s11(xb), se = mueller_s11(xe);
+                        v[0] = fma(w1, m[0], v[0]);
+                        v[1] = fma(w2, m[0], v[1]);
+                        v[2] = fma(w1, sb, v[2]);
+                        v[3] = fma(w2, sb, v[3]);
+                        v[4] = fma(w1, se, v[4]);
+                        v[5] = fma(w2, se, v[5]);
+                    }
                 }
             }
+            if (!LIST) {
 #pragma unroll
-            for (int q = 0; q < 6; q++) v[q] = wave_sum(v[q]);
-            if ((tid & 63) == 0) {
+                for (int q = 0; q < 6; q++) v[q] = wave_sum(v[q]);
+                if ((tid & 63) == 0) {
 #pragma unroll
-                for (int q = 0; q < 6; q++) red[tid >> 6][q] = v[q];
+                    for (int q = 0; q < 6; q++) red[tid >> 6][q] = v[q];
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double* ip = a.ipart + ((int64_t)o * a.ntiles + t) * 6;
+#pragma unroll
+                    for (int q = 0; q < 6; q++) ip[q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (tid == 0) {
-                double* ip = a.ipart + ((int64_t)o * a.ntiles + t) * 6;
-#pragma unroll
-                for (int q = 0; q < 6; q++) ip[q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
-            }
-            __syncthreads();
         }
     }
 }
@@ -264,8 +300,77 @@ __global__ __launch_bounds__(BB) void k_batch_phi(const double* __restrict__ M, 
 
 __global__ void k_zero_u32(unsigned int* p) { if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0u; }
 
-template <int PREC, int TB, int BPT>
-const void* batch_kernel() { return reinterpret_cast<const void*>(&k_batch<PREC, TB, BPT>); }
+// ---- gated mixed precision ---------------------------------------------------
+//
+// The FP32 body's rounding makes each orientation's amplitude carry a relative error of
+// ~1e-7 of the bin's incoherent beam sum; summed over orientations with independent
+// signs, the error of a Mueller element of the orientation sum is bounded by
+// EST_C * sqrt(sum_o S11_o I_o) (calibrated on the rough-column and absorbing-column
+// workloads against the FP64 reference, with a margin).  A bin is recomputed in FP64 when
+// that bound exceeds the accuracy gate of one of its elements:
+//   |dS_11| <= GATE_S11 S11,   |dS_ij| <= GATE_ELEM max(|S_ij|, GATE_FLOOR S11).
+constexpr double EST_C = 5.0e-7;
+constexpr double GATE_S11 = 1.0e-6;
+constexpr double GATE_ELEM = 1.0e-5;
+constexpr double GATE_FLOOR = 1.0e-3;
+
+__global__ __launch_bounds__(BB) void k_gate_flags(const double* __restrict__ M, const double* __restrict__ est,
+                                                   int nbins, int32_t* __restrict__ flags)
+{
+    const int b = blockIdx.x * BB + threadIdx.x;
+    if (b >= nbins) return;
+    const double s11 = fabs(M[b]);
+    double mn = s11;
+    for (int e = 1; e < 16; e++) mn = fmin(mn, fabs(M[(int64_t)e * nbins + b]));
+    const double bound = EST_C * sqrt(fmax(est[b], 0.0));
+    const double tol = fmin(GATE_S11 * s11, GATE_ELEM * fmax(mn, GATE_FLOOR * s11));
+    // NaN anywhere (bound or tolerance) also sends the bin to FP64
+    flags[b] = !(bound <= tol);
+}
+
+// Flagged bins in ascending order: one workgroup, each lane a contiguous segment.
+constexpr int CB = 1024;
+__global__ __launch_bounds__(CB) void k_gate_compact(const int32_t* __restrict__ flags, int nbins,
+                                                     int32_t* __restrict__ list, int32_t* __restrict__ count)
+{
+    __shared__ int32_t sc[CB];
+    const int tid = threadIdx.x;
+    const int per = (nbins + CB - 1) / CB;
+    const int b0 = min(nbins, tid * per), b1 = min(nbins, b0 + per);
+    int n = 0;
+    for (int b = b0; b < b1; b++) n += flags[b];
+    sc[tid] = n;
+    __syncthreads();
+    for (int off = 1; off < CB; off <<= 1) {   // inclusive scan
+        const int v = tid >= off ? sc[tid - off] : 0;
+        __syncthreads();
+        sc[tid] += v;
+        __syncthreads();
+    }
+    int pos = sc[tid] - n;
+    for (int b = b0; b < b1; b++)
+        if (flags[b]) list[pos++] = b;
+    if (tid == CB - 1) *count = sc[CB - 1];
+}
+
+// mueller[e][list[i]] = sum over chunks of the list pass's partials (chunk order)
+__global__ __launch_bounds__(BB) void k_gate_scatter(const double* __restrict__ mpart, int nchunks, int nbins,
+                                                     const int32_t* __restrict__ list,
+                                                     const int32_t* __restrict__ count, double* __restrict__ out)
+{
+    const int64_t idx = (int64_t)blockIdx.x * BB + threadIdx.x;   // over 16 * nbins
+    const int e = (int)(idx / nbins);
+    const int i = (int)(idx - (int64_t)e * nbins);
+    if (e >= 16 || i >= *count) return;
+    const int64_t stride = (int64_t)16 * nbins;
+    const double* p = mpart + (int64_t)e * nbins + i;
+    double s = p[0];
+    for (int c = 1; c < nchunks; c++) s += p[c * stride];
+    out[(int64_t)e * nbins + list[i]] = s;
+}
+
+template <int PREC, int TB, int BPT, int MODE>
+const void* batch_kernel() { return reinterpret_cast<const void*>(&k_batch<PREC, TB, BPT, MODE>); }
 
 }  // namespace abt
 
@@ -273,33 +378,90 @@ const void* batch_kernel() { return reinterpret_cast<const void*>(&k_batch<PREC,
 // launch helpers (called by the runtime)
 using abt::BatchArgs;
 
-extern "C" const void* abt_batch_kernel(int prec, int tb, int bpt)
+extern "C" const void* abt_batch_kernel(int prec, int tb, int bpt, int mode)
 {
     using namespace abt;
-#define K(P, T, B) if (prec == P && tb == T && bpt == B) return batch_kernel<P, T, B>();
-    K(0, 8, 1) K(0, 16, 1) K(0, 32, 1) K(0, 8, 2) K(0, 16, 2) K(0, 32, 2)
-    K(1, 8, 1) K(1, 16, 1) K(1, 32, 1) K(1, 8, 2) K(1, 16, 2) K(1, 32, 2)
-    K(3, 16, 1)
+#define K(P, T, B, M) if (prec == P && tb == T && bpt == B && mode == M) return batch_kernel<P, T, B, M>();
+    K(0, 8, 1, 0) K(0, 16, 1, 0) K(0, 32, 1, 0) K(0, 8, 2, 0) K(0, 16, 2, 0) K(0, 32, 2, 0)
+    K(1, 8, 1, 0) K(1, 16, 1, 0) K(1, 32, 1, 0) K(1, 8, 2, 0) K(1, 16, 2, 0) K(1, 32, 2, 0)
+    K(1, 8, 1, 1) K(1, 16, 1, 1) K(1, 32, 1, 1) K(1, 8, 2, 1) K(1, 16, 2, 1) K(1, 32, 2, 1)
+    K(3, 16, 1, 0)
+    K(4, 8, 1, 0) K(4, 16, 1, 0) K(4, 32, 1, 0)
+    K(4, 16, 1, 2)
 #undef K
     return nullptr;
+}
+
+namespace {
+int launch_pass(const void* kern, const BatchArgs& a, int wgs, hipStream_t st)
+{
+    hipLaunchKernelGGL(abt::k_zero_u32, dim3(1), dim3(64), 0, st, a.counter);
+    BatchArgs args = a;
+    void* params[] = {&args};
+    return (int)hipLaunchKernel(kern, dim3(wgs), dim3(abt::BB), params, 0, st);
+}
+}  // namespace
+
+extern "C" int abt_launch_batch_pass(const BatchArgs* a, int prec, int tb, int bpt, int mode, int wgs,
+                                     hipStream_t st)
+{
+    const void* kern = abt_batch_kernel(prec, tb, bpt, mode);
+    if (!kern) return (int)hipErrorInvalidValue;
+    int e = launch_pass(kern, *a, wgs, st);
+    return e ? e : (int)hipGetLastError();
 }
 
 extern "C" int abt_launch_batch(const BatchArgs* a, int prec, int tb, int bpt, int wgs, int norient,
                                 double* mueller, double* mueller_1d, double* integrals, hipStream_t st)
 {
     using namespace abt;
-    const void* kern = abt_batch_kernel(prec, tb, bpt);
+    const void* kern = abt_batch_kernel(prec, tb, bpt, BMODE_PLAIN);
     if (!kern) return (int)hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, a->counter);
-    BatchArgs args = *a;
-    void* params[] = {&args};
-    hipError_t e = hipLaunchKernel(kern, dim3(wgs), dim3(BB), params, 0, st);
-    if (e != hipSuccess) return (int)e;
+    int e = launch_pass(kern, *a, wgs, st);
+    if (e) return e;
     const int64_t n = (int64_t)16 * a->g.nbins;
     hipLaunchKernelGGL(k_batch_reduce_m, dim3((unsigned)((n + BB - 1) / BB)), dim3(BB), 0, st,
                        (const double*)a->mpart, a->nchunks, n, mueller);
     hipLaunchKernelGGL(k_batch_reduce_i, dim3((unsigned)((norient * 6 + BB - 1) / BB)), dim3(BB), 0, st,
                        (const double*)a->ipart, a->ntiles, norient, integrals);
+    if (mueller_1d)
+        hipLaunchKernelGGL(k_batch_phi, dim3((unsigned)((16 * a->g.ntheta + BB - 1) / BB)), dim3(BB), 0, st,
+                           (const double*)mueller, a->g.phi, a->g.nphi, a->g.ntheta, mueller_1d);
+    return (int)hipGetLastError();
+}
+
+extern "C" int abt_launch_batch_gated(const BatchArgs* a, int tb, int bpt, int wgs, int wgs2, int norient,
+                                      double* est, int32_t* flags, int32_t* list, int32_t* count,
+                                      double* mueller, double* mueller_1d, double* integrals, hipStream_t st)
+{
+    using namespace abt;
+    const void* k1 = abt_batch_kernel(1, tb, bpt, BMODE_EST);
+    const void* k2 = abt_batch_kernel(4, 16, 1, BMODE_LIST);
+    if (!k1 || !k2) return (int)hipErrorInvalidValue;
+    const int nbins = a->g.nbins;
+    // pass 1: mixed, with the error scale
+    int e = launch_pass(k1, *a, wgs, st);
+    if (e) return e;
+    const int64_t n = (int64_t)16 * nbins;
+    hipLaunchKernelGGL(k_batch_reduce_m, dim3((unsigned)((n + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->mpart, a->nchunks, n, mueller);
+    hipLaunchKernelGGL(k_batch_reduce_m, dim3((unsigned)((nbins + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->epart, a->nchunks, (int64_t)nbins, est);
+    hipLaunchKernelGGL(k_batch_reduce_i, dim3((unsigned)((norient * 6 + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->ipart, a->ntiles, norient, integrals);
+    // the bins whose error bound exceeds the gate
+    hipLaunchKernelGGL(k_gate_flags, dim3((unsigned)((nbins + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)mueller, (const double*)est, nbins, flags);
+    hipLaunchKernelGGL(k_gate_compact, dim3(1), dim3(CB), 0, st, (const int32_t*)flags, nbins, list, count);
+    // pass 2: those bins in FP64 (restructured body), written over the mixed sums
+    BatchArgs b = *a;
+    b.list = list;
+    b.list_count = count;
+    e = launch_pass(k2, b, wgs2, st);
+    if (e) return e;
+    hipLaunchKernelGGL(k_gate_scatter, dim3((unsigned)((n + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->mpart, a->nchunks, nbins, (const int32_t*)list, (const int32_t*)count,
+                       mueller);
     if (mueller_1d)
         hipLaunchKernelGGL(k_batch_phi, dim3((unsigned)((16 * a->g.ntheta + BB - 1) / BB)), dim3(BB), 0, st,
                            (const double*)mueller, a->g.phi, a->g.nphi, a->g.ntheta, mueller_1d);

@@ -48,19 +48,40 @@ void make_chunks(const int64_t* boff, const int64_t* eoff, int n, int nch, std::
     out.push_back(n);
 }
 
-// ABI precision -> kernel variant (ABT_FP64_ACCURATE runs the accurate-phase FP64 body)
-int kernel_prec(int precision) { return precision == ABT_FP64_ACCURATE ? 3 : precision; }
-
-bool shape_allowed(int prec, int tb, int bpt) { return prec != ABT_FP64_ACCURATE || (tb == 16 && bpt == 1); }
-
-int occupancy(int prec, int tb, int bpt, int* per_cu)
+// ABI precision -> kernel variant: ABT_FP64_ACCURATE runs the accurate-phase FP64 body (3),
+// ABT_FP64_FAST the restructured FP64 body (4); ABT_MIXED_GATED's tuned pass is the mixed
+// body with its error scale
+int kernel_prec(int precision)
 {
-    const void* k = abt_batch_kernel(kernel_prec(prec), tb, bpt);
-    if (!k) return abt_fail(ABT_ERR_ARG, "no batched kernel for beams_per_tile %d, bins_per_thread %d", tb, bpt);
+    switch (precision) {
+        case ABT_FP64_ACCURATE: return 3;
+        case ABT_FP64_FAST: return 4;
+        case ABT_MIXED_GATED: return 1;
+        default: return precision;
+    }
+}
+int kernel_mode(int precision) { return precision == ABT_MIXED_GATED ? abt::BMODE_EST : abt::BMODE_PLAIN; }
+
+bool shape_allowed(int prec, int tb, int bpt)
+{
+    if (prec == ABT_FP64_ACCURATE) return tb == 16 && bpt == 1;
+    if (prec == ABT_FP64_FAST) return bpt == 1;
+    return true;
+}
+
+int kernel_occupancy(const void* k, int* per_cu)
+{
     int n = 0;
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, LANES, 0));
     *per_cu = std::max(1, n);
     return 0;
+}
+
+int occupancy(int prec, int tb, int bpt, int* per_cu)
+{
+    const void* k = abt_batch_kernel(kernel_prec(prec), tb, bpt, kernel_mode(prec));
+    if (!k) return abt_fail(ABT_ERR_ARG, "no batched kernel for beams_per_tile %d, bins_per_thread %d", tb, bpt);
+    return kernel_occupancy(k, per_cu);
 }
 
 }  // namespace
@@ -82,6 +103,13 @@ struct abt_batch_plan {
     double* d_mpart = nullptr;       // [max_chunks][16][nbins]
     double* d_ipart = nullptr;       // [norient][ntiles(bpt=1)][6]
     unsigned int* d_counter = nullptr;
+    // gated mixed precision: error-scale partials and sums, flags, the list of FP64 bins
+    double* d_epart = nullptr;       // [max_chunks][nbins]
+    double* d_est = nullptr;         // [nbins]
+    int32_t* d_flags = nullptr;      // [nbins]
+    int32_t* d_list = nullptr;       // [nbins]
+    int32_t* d_count = nullptr;
+    int wgs_list = 0;                // persistent workgroups of the FP64 list pass
     int max_chunks = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -123,6 +151,9 @@ void fill_args(const abt_batch_plan* p, const Shape& s, const int32_t* chunk, co
     a->ipart = p->d_ipart;
     a->counter = p->d_counter;
     a->ampl_out = nullptr;
+    a->epart = p->d_epart;
+    a->list = p->d_list;
+    a->list_count = p->d_count;
     a->nchunks = s.nchunks;
     a->ntiles = s.ntiles;
     a->nunits = s.nchunks * s.ntiles;
@@ -164,7 +195,8 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
 {
     if (!g || !beam_off || !ext_off || !out) return abt_fail(ABT_ERR_ARG, "abt_batch_plan_create: null argument");
     if (norient < 1) return abt_fail(ABT_ERR_ARG, "abt_batch_plan_create: need at least one orientation");
-    if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_ACCURATE)
+    if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_ACCURATE &&
+        precision != ABT_FP64_FAST && precision != ABT_MIXED_GATED)
         return abt_fail(ABT_ERR_ARG, "unknown precision %d", precision);
     if (g->d.nphi < 3 || g->d.ntheta < 3)
         return abt_fail(ABT_ERR_ARG, "the phi and theta integrals (simpne) need at least 3 nodes each");
@@ -195,6 +227,13 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
         }
     rc = shape_for(p, norient, 16, 1, &p->shape);
     if (rc) { delete p; return rc; }
+    const bool gated = precision == ABT_MIXED_GATED;
+    if (gated) {
+        int per_cu = 1;
+        rc = kernel_occupancy(abt_batch_kernel(4, 16, 1, abt::BMODE_LIST), &per_cu);
+        if (rc) { delete p; return rc; }
+        p->wgs_list = per_cu * cus;
+    }
     const int64_t nbins = g->nbins, ntiles1 = (nbins + LANES - 1) / LANES;
     const int nth = g->d.ntheta, nph = g->d.nphi;
     size_t off = 0;
@@ -208,6 +247,11 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
     const size_t o_mp = take(sizeof(double) * 16 * nbins * (size_t)p->max_chunks);
     const size_t o_ip = take(sizeof(double) * 6 * (size_t)ntiles1 * norient);
     const size_t o_ct = take(sizeof(unsigned int) * 64);
+    const size_t o_ep = gated ? take(sizeof(double) * nbins * (size_t)p->max_chunks) : 0;
+    const size_t o_est = gated ? take(sizeof(double) * nbins) : 0;
+    const size_t o_fl = gated ? take(sizeof(int32_t) * nbins) : 0;
+    const size_t o_li = gated ? take(sizeof(int32_t) * nbins) : 0;
+    const size_t o_cn = gated ? take(sizeof(int32_t) * 64) : 0;
     if (hipMalloc(&p->mem, off) != hipSuccess) {
         delete p;
         return abt_fail(ABT_ERR_NOMEM, "batched plan: hipMalloc of %zu bytes failed", off);
@@ -222,6 +266,13 @@ int abt_batch_plan_create(const abt_dev_grid* g, int32_t norient, const int64_t*
     p->d_mpart = (double*)(m + o_mp);
     p->d_ipart = (double*)(m + o_ip);
     p->d_counter = (unsigned int*)(m + o_ct);
+    if (gated) {
+        p->d_epart = (double*)(m + o_ep);
+        p->d_est = (double*)(m + o_est);
+        p->d_flags = (int32_t*)(m + o_fl);
+        p->d_list = (int32_t*)(m + o_li);
+        p->d_count = (int32_t*)(m + o_cn);
+    }
     // integral weights: simpne over phi, then x sin(theta) / k^2 (x cos(theta)) and simpne over theta
     std::vector<double> th(nth), ph(nph), wt(nth), wp(nph), w1(nbins), w2(nbins);
     hipError_t e = hipMemcpy(th.data(), g->d.theta, sizeof(double) * nth, hipMemcpyDeviceToHost);
@@ -330,8 +381,12 @@ int abt_batch_autotune(abt_batch_plan* p, const double* recs_b, const double* vr
             float ms = 0.f;
             for (int rep = 0; rep < 2 && !rc; rep++) {   // first launch warms up
                 he = hipEventRecord(e0, st);
-                int le = abt_launch_batch(&a, kernel_prec(p->prec), tb, bpt, s.wgs, ns, out, nullptr,
-                                          out + 16 * nbins, st);
+                // gated plans tune their mixed pass (the plain mixed kernel of the same shape
+                // plus the error scale: run the pass alone)
+                int le = kernel_mode(p->prec) == abt::BMODE_EST
+                             ? abt_launch_batch_pass(&a, kernel_prec(p->prec), tb, bpt, kernel_mode(p->prec), s.wgs, st)
+                             : abt_launch_batch(&a, kernel_prec(p->prec), tb, bpt, s.wgs, ns, out, nullptr,
+                                                out + 16 * nbins, st);
                 if (le) rc = abt_fail(ABT_ERR_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)le));
                 if (he == hipSuccess) he = hipEventRecord(e1, st);
                 if (he == hipSuccess) he = hipEventSynchronize(e1);
@@ -371,7 +426,13 @@ int abt_batch_run(const abt_batch_plan* p, const double* recs_b, const double* v
     abt::BatchArgs a;
     fill_args(p, p->shape, p->d_chunk, recs_b, vrecs_b, recs_e, vrecs_e, &a);
     a.ampl_out = dev_ampl;
-    int e = abt_launch_batch(&a, kernel_prec(p->prec), p->shape.tb, p->shape.bpt, p->shape.wgs, p->norient,
+    int e;
+    if (p->prec == ABT_MIXED_GATED)
+        e = abt_launch_batch_gated(&a, p->shape.tb, p->shape.bpt, p->shape.wgs, p->wgs_list, p->norient,
+                                   p->d_est, p->d_flags, p->d_list, p->d_count, dev_mueller, dev_mueller_1d,
+                                   dev_integrals, (hipStream_t)stream);
+    else
+        e = abt_launch_batch(&a, kernel_prec(p->prec), p->shape.tb, p->shape.bpt, p->shape.wgs, p->norient,
                              dev_mueller, dev_mueller_1d, dev_integrals, (hipStream_t)stream);
     if (e) return abt_fail(ABT_ERR_HIP, "batched launch failed: %s", hipGetErrorString((hipError_t)e));
     return 0;
@@ -421,6 +482,21 @@ int abt_batch_workspace(const abt_batch_plan* p, size_t* bytes)
     const int64_t nbins = p->g->nbins, ntiles1 = (nbins + LANES - 1) / LANES;
     *bytes = sizeof(double) * (16 * (size_t)nbins * p->max_chunks + 6 * (size_t)ntiles1 * p->norient +
                                2 * (size_t)nbins);
+    if (p->prec == ABT_MIXED_GATED)
+        *bytes += sizeof(double) * ((size_t)nbins * p->max_chunks + nbins) + sizeof(int32_t) * 2 * (size_t)nbins;
+    return 0;
+}
+
+int abt_batch_gate_stats(const abt_batch_plan* p, int32_t* refined_bins, void* stream)
+{
+    if (!p || !refined_bins) return abt_fail(ABT_ERR_ARG, "abt_batch_gate_stats: null argument");
+    if (p->prec != ABT_MIXED_GATED) {
+        *refined_bins = 0;
+        return 0;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(hipMemcpyAsync(refined_bins, p->d_count, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return 0;
 }
 

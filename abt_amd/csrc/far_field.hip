@@ -12,6 +12,7 @@
 // k_ff_fp64  -- the reference's formulas in FP64, operation for operation
 //               (diffraction, diff_mod.f90:576-670; karczewski :124-198;
 //               contour_integral :357-421), FMA contraction off.
+// k_ff_fp64r -- (PREC 4) the mixed kernel's algorithm below with every step in FP64.
 // k_ff_mixed -- the same quantity restructured for the vector ALUs:
 //               * geometry, phases and their reduction to octants in FP64
 //                 (the bin direction, q = k (prop - khat), vertex phases,
@@ -44,7 +45,7 @@ constexpr int BLOCK = 256;
 // The mixed kernel with up to 16-beam tiles is held to 128 VGPRs (4 waves per SIMD), its
 // fastest occupancy on gfx950.
 template <int PREC, int TB>
-__global__ __launch_bounds__(BLOCK, (PREC == 1 && TB <= 16) ? 4 : 1)
+__global__ __launch_bounds__(BLOCK, (PREC == 1 && TB <= 16) ? 4 : (PREC == 4 ? 2 : 1))
 void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __restrict__ vrecs,
                  int64_t nb, int64_t per_chunk, int fast_diff, double* __restrict__ out,
                  double* __restrict__ work)
@@ -54,8 +55,11 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
     // tile).  The FP64 bodies need their registers: one tile, loaded between barriers.
     constexpr bool DB = PREC == 1;
     __shared__ double tile[(DB ? 2 : 1) * TB * REC];
+    __shared__ double etab[PREC == 4 ? ETAB_DOUBLES : 1];   // exponential table of the FP64r body
     constexpr int PF = DB ? (TB * REC + BLOCK - 1) / BLOCK : 1;   // doubles per lane per tile
     const int tid = threadIdx.x;
+    if (PREC == 4)   // published by the barriers of the tile loop before any use
+        for (int q = tid; q < ETAB_DOUBLES; q += BLOCK) etab[q] = g.etab[q];
     const int bin = blockIdx.x * BLOCK + tid;
     const bool active = bin < g.nbins;
     const int bc = active ? bin : g.nbins - 1;
@@ -112,6 +116,8 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
 #pragma unroll
                     for (int q = 0; q < 8; q++) acc[q] += (double)accf[q];
                 }
+            } else if (PREC == 4) {
+                for (int bb = 0; bb < nt; bb++) beam_bin_fp64r(cur + bb * REC, vrecs, g, bi, etab, fast_diff, acc);
             } else {
                 for (int bb = 0; bb < nt; bb++)
                     beam_bin_fp64(cur + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, PREC == 2);
@@ -129,7 +135,7 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
         }
     }
     if (!active) return;
-    if (PREC == 1) mixed_finish(acc, cp, sp);
+    if (PREC == 1 || PREC == 4) mixed_finish(acc, cp, sp);
     const int nbins = g.nbins;
     if (PREC == 2) {   // incoherent intensity: one double per bin, never chunked
         out[bin] = acc[0];
@@ -201,6 +207,8 @@ extern "C" int abt_launch_far_field(const abt::DevGrid* g, const double* recs, c
         if (tb == 8) L(0, 8); else if (tb == 32) L(0, 32); else L(0, 16);
     } else if (prec == 1) {
         if (tb == 8) L(1, 8); else if (tb == 32) L(1, 32); else L(1, 16);
+    } else if (prec == 3) {   // ABT_FP64_FAST: the restructured FP64 body
+        L(4, 16);
     } else {
         L(2, 16);
     }

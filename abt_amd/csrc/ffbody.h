@@ -308,11 +308,13 @@ __host__ __device__ __forceinline__ void edge_exp(float wr0, float wi0, float wr
 
 // Mixed-precision body: one beam, one bin.  Adds the beam's contribution to the FP32
 // projector sums accf: S+ (accf[0..3]) and S- (accf[4..7]), see mixed_finish; callers
-// add accf into FP64 sums every few beams.
-__host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict__ r,
-                                                        const double* __restrict__ vrecs,
-                                                        const DevGrid& g, const BinIn& bi,
-                                                        int fast_diff, float accf[8])
+// add accf into FP64 sums every few beams.  INC: also adds the squared Frobenius norm of
+// the beam's amplitude, |f|^2 (gg^2 + hh^2) |A|_F^2 (the incoherent sum), to inc.
+template <bool INC>
+__host__ __device__ __forceinline__ void beam_bin_mixed_t(const double* __restrict__ r,
+                                                          const double* __restrict__ vrecs,
+                                                          const DevGrid& g, const BinIn& bi,
+                                                          int fast_diff, float accf[8], float& inc)
 {
     const float* rf = reinterpret_cast<const float*>(r + R_F32);
     const float* rg = reinterpret_cast<const float*>(r + R_G32);
@@ -501,6 +503,7 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     // w' = f (gg - i hh);  P A = [r+; i r+],  Q A = [r-; -i r-]
     const float ta = fR * gg, tb = fI * hh, tc = fR * hh, td = fI * gg;
     const float wr = ta - tb, wi = tc + td, vr_ = ta + tb, vi = td - tc;
+    if (INC) inc = fmaf(fmaf(fR, fR, fI * fI) * fmaf(gg, gg, hh * hh), rf[F_C2], inc);
     const float* rp = rg + G_RP;
     const float* rm = rg + G_RM;
     accf[0] = fmaf(wr, rp[0], fmaf(-wi, rp[1], accf[0]));
@@ -511,6 +514,286 @@ __host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict
     accf[5] = fmaf(vr_, rm[1], fmaf(vi, rm[0], accf[5]));
     accf[6] = fmaf(vr_, rm[2], fmaf(-vi, rm[3], accf[6]));
     accf[7] = fmaf(vr_, rm[3], fmaf(vi, rm[2], accf[7]));
+}
+
+__host__ __device__ __forceinline__ void beam_bin_mixed(const double* __restrict__ r,
+                                                        const double* __restrict__ vrecs,
+                                                        const DevGrid& g, const BinIn& bi,
+                                                        int fast_diff, float accf[8])
+{
+    float unused = 0.f;
+    beam_bin_mixed_t<false>(r, vrecs, g, bi, fast_diff, accf, unused);
+}
+
+// ---------------------------------------------------------------------------
+// Restructured FP64 body: the mixed body's algorithm (bin direction from the aperture
+// centre, rotation-invariant edge sum from vertex half-phase exponentials, the hc-normal
+// polarisation closed form, projector accumulation) with every step in FP64.  Agrees with
+// the reference's formula to ~1e-12 of the largest amplitude (the reference's own direct
+// phase k |x3| ~ 1e7 rad carries ~1e-9 rad of rounding), at a fraction of its cost: three
+// vertex exponentials per beam and bin instead of the split sum's twelve trigonometric
+// pairs.  Used where the FP32 body's error would exceed the accuracy gate (the refinement
+// pass of the mixed kernels) and as a fast FP64 precision of its own.
+
+constexpr double PI64 = 3.141592653589793;
+
+// reciprocal and reciprocal square root: hardware approximations refined by Newton steps
+__host__ __device__ __forceinline__ double rcp64(double x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, y, 1.0);
+    y = fma(y, e, y);
+    e = fma(-x, y, 1.0);
+    return fma(y, e, y);
+#else
+    return 1.0 / x;
+#endif
+}
+__host__ __device__ __forceinline__ double rsq64(double x)
+{
+    double y = (double)abt_rsqf((float)x);
+    double e = fma(-x * y, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+
+// reciprocal and reciprocal square root to ~1e-14: the FP32 hardware approximation and
+// one Newton step in FP64 (inputs within the FP32 range)
+__host__ __device__ __forceinline__ double rcp64_1(double x)
+{
+    const double y = (double)abt_rcpf((float)x);
+    return fma(y, fma(-x, y, 1.0), y);
+}
+__host__ __device__ __forceinline__ double rsq64_1(double x)
+{
+    const double y = (double)abt_rsqf((float)x);
+    return fma(0.5 * y, fma(-x * y, y, 1.0), y);
+}
+
+// The table of e^{i pi j / ETAB_N} (DevGrid::etab), built by the host.
+inline void fill_exp_table(double* tab)
+{
+    for (int j = -ETAB_N; j <= ETAB_N; j++) {
+        const double a = PI64 * (double)j / (double)ETAB_N;
+        tab[2 * (j + ETAB_N)] = cos(a);
+        tab[2 * (j + ETAB_N) + 1] = sin(a);
+    }
+}
+
+// e^{i pi t} for an FP64 t (half revolutions): t = 2m + j / ETAB_N + r with |r| <= 1 / (2 ETAB_N)
+// (every step exact), e^{i pi t} = tab[j] e^{i pi r}, the last from its Taylor polynomials
+// (|pi r| <= 3.1e-3: truncation below 2e-18).  tab: the exponential table (in LDS on the device).
+__host__ __device__ __forceinline__ void expi_pi64(double t, const double* __restrict__ tab, double* c, double* s)
+{
+    const double m = rint(0.5 * t);
+    const double u = fma(-2.0, m, t);                       // [-1, 1]
+    const double jf = rint((double)ETAB_N * u);
+    const double r = fma(-1.0 / (double)ETAB_N, jf, u);     // |r| <= 1 / (2 ETAB_N)
+    const int j = (int)jf + ETAB_N;
+    const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
+    const double y = PI64 * r;
+    const double z = y * y;
+    const double sr = fma(y * z, fma(z, 1.0 / 120.0, -1.0 / 6.0), y);
+    const double cr = fma(z, fma(z, 1.0 / 24.0, -0.5), 1.0);
+    *c = fma(c0, cr, -s0 * sr);
+    *s = fma(s0, cr, c0 * sr);
+}
+
+// one edge of the rotation-invariant form factor in FP64 (see edge_exp): c_j sinc(pi d) w_j w_{j+1}
+__host__ __device__ __forceinline__ void edge_exp64(double wr0, double wi0, double wr1, double wi1, double d,
+                                                    double qx, double qy, double edx, double edy, double& ssr,
+                                                    double& ssi)
+{
+    const double Er = fma(wr0, wr1, -wi0 * wi1);
+    const double Ei = fma(wr0, wi1, wi0 * wr1);
+    const double sx = fma(wi1, wr0, -wr1 * wi0);   // sin(pi d)
+    const double pd = PI64 * d;
+    const double z = pd * pd;
+    // sin(x)/x for |x| < pi/64 (truncation < 1e-16); sin(pi d) / (pi d) elsewhere, whose
+    // relative error is below 4e-15 there
+    const double sp = fma(z, fma(z, fma(z, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), 1.0);
+    const double S = fabs(d) < 1.0 / 64.0 ? sp : sx * rcp64_1(pd);
+    const double t = fma(qx, edy, -qy * edx) * S;
+    ssr = fma(t, Er, ssr);
+    ssi = fma(t, Ei, ssi);
+}
+
+// Restructured FP64 body: one beam, one bin; adds to the projector sums S+ (acc[0..3]) and
+// S- (acc[4..7]) that mixed_finish turns into the amplitude matrix.
+__host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict__ r,
+                                                        const double* __restrict__ vrecs,
+                                                        const DevGrid& g, const BinIn& bi,
+                                                        const double* __restrict__ etab,
+                                                        int fast_diff, double acc[8])
+{
+    // ---- geometry: bin direction from the aperture centre, |r - com| ----
+    const double dx = bi.x - r[R_COM], dy = bi.y - r[R_COM + 1], dz = bi.z - r[R_COM + 2];
+    const double s = fma(dx, dx, fma(dy, dy, dz * dz));
+    const double invd = rsq64(s);
+    const double ux = dx * invd, uy = dy * invd, uz = dz * invd;
+    if (fast_diff) {
+        if (ux * r[R_PROP0] + uy * r[R_PROP0 + 1] + uz * r[R_PROP0 + 2] < r[R_COSFOV]) return;
+    }
+    const double* K = r + R_KRR;
+    const double qx = fma(-K[0], ux, fma(-K[1], uy, fma(-K[2], uz, r[R_KP2])));
+    const double qy = fma(-K[3], ux, fma(-K[4], uy, fma(-K[5], uz, r[R_KP2 + 1])));
+    const double k1r = r[R_KP2] - qx, k2r = r[R_KP2 + 1] - qy;
+    const double kd = g.krev * (s * invd);
+    const double q2 = fma(qx, qx, qy * qy);
+    const double lam = g.lambda;
+
+    // ---- aperture integral with the far-field phase: (FR, FI) = e^{i k|r-com|} sum_ref / lambda ----
+    double FR, FI;
+    const double* rf_ = r;   // vertex coordinates of triangles are inline
+    const bool tri = r[R_NV] == 3.0;
+    const double qa2 = (double)reinterpret_cast<const float*>(r + R_F32)[F_QA2];
+    if (q2 * qa2 < (double)SERIES_QA2) {
+        const double tw = 6.283185307179586;
+        double sr = 0.0, si = 0.0;
+        if (tri) {
+            const double a = tw * fma(qx, rf_[R_EX], qy * rf_[R_EY]);
+            const double b = tw * fma(qx, rf_[R_EX + 1], qy * rf_[R_EY + 1]);
+            const double c = tw * fma(qx, rf_[R_EX + 2], qy * rf_[R_EY + 2]);
+            double tr, ti;
+            tri_series(a, b, c, &tr, &ti);
+            sr = r[R_AREA] * tr;
+            si = r[R_AREA] * ti;
+        } else {
+            const int nv = (int)r[R_NV];
+            const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
+            for (int j = 0; j < nv; j++) {
+                const int jp = (j == nv - 1) ? 0 : j + 1;
+                const double xj = vr[VREC * j + V_X], yj = vr[VREC * j + V_Y];
+                const double xn = vr[VREC * jp + V_X], yn = vr[VREC * jp + V_Y];
+                const double a = tw * fma(qx, xj, qy * yj);
+                const double b = tw * fma(qx, xn, qy * yn);
+                const double sa = 0.5 * (xj * yn - xn * yj);
+                double tr, ti;
+                tri_series(0.0, a, b, &tr, &ti);
+                sr += sa * tr;
+                si += sa * ti;
+            }
+        }
+        // sum_ref = -i F = si - i sr
+        double cb, sb;
+        expi_pi64(2.0 * kd, etab, &cb, &sb);
+        const double il = rcp64(lam);
+        FR = (cb * si + sb * sr) * il;
+        FI = (sb * si - cb * sr) * il;
+    } else {
+        double ssr = 0.0, ssi = 0.0;
+        if (tri) {
+            double t[3], wr[3], wi[3];
+#pragma unroll
+            for (int l = 0; l < 3; l++) {
+                t[l] = fma(qx, r[R_EX + l], fma(qy, r[R_EY + l], kd));
+                expi_pi64(t[l], etab, &wr[l], &wi[l]);
+            }
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int jn = j == 2 ? 0 : j + 1;
+                edge_exp64(wr[j], wi[j], wr[jn], wi[jn], t[jn] - t[j], qx, qy, r[R_EDX + j], r[R_EDY + j], ssr, ssi);
+            }
+        } else {
+            const int nv = (int)r[R_NV];
+            const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
+            const double t0 = fma(qx, vr[V_X], fma(qy, vr[V_Y], kd));
+            double wr0, wi0;
+            expi_pi64(t0, etab, &wr0, &wi0);
+            double tp = t0, wrp = wr0, wip = wi0;
+            for (int j = 0; j < nv; j++) {
+                double tn = t0, wrn = wr0, win = wi0;
+                if (j < nv - 1) {
+                    tn = fma(qx, vr[VREC * (j + 1) + V_X], fma(qy, vr[VREC * (j + 1) + V_Y], kd));
+                    expi_pi64(tn, etab, &wrn, &win);
+                }
+                edge_exp64(wrp, wip, wrn, win, tn - tp, qx, qy, vr[VREC * j + V_DX], vr[VREC * j + V_DY], ssr, ssi);
+                tp = tn;
+                wrp = wrn;
+                wip = win;
+            }
+        }
+        const double sc = -rcp64_1(6.283185307179586 * lam * q2);
+        FR = ssr * sc;
+        FI = ssi * sc;
+    }
+
+    // ---- polarisation (karczewski + rot4, diff_mod.f90:124-198, 630-643) ----
+    const bool kclamp = fabs(k2r) > g.krev_k2clamp;
+    const bool fwd = !(fabs(uz) < FWD_DOT);
+    const double ik = g.inv_krev;
+    const double k1 = k1r * ik, k2 = k2r * ik;
+    const double* RR = r + R_RR;
+    const double k3 = fma(RR[6], ux, fma(RR[7], uy, RR[8] * uz));
+    const double s2 = fma(k1, k1, k3 * k3);                     // 1 - k_y^2 (unclamped)
+    const double* C = r + R_C64;
+    double gg, hh;
+    if (!kclamp && !fwd) {
+        const double P0 = fma(K[3], uy, -K[4] * ux);
+        const double Q0 = fma(-uz, k2r, K[5]);
+        const double rh = rsq64_1(fma(ux, ux, uy * uy)) * rcp64_1(s2);
+        const double P = P0 * rh, Q = Q0 * rh;
+        const double A = fma(C[0], s2, C[1] * k3);
+        const double B = fma(C[2], s2, C[1] * (k1 * k2));
+        gg = fma(P, A, Q * B);
+        hh = fma(P, B, -Q * A);
+    } else {
+        const double Kx = r[R_PROP2], Ky = r[R_PROP2 + 1], Kz = r[R_PROP2 + 2];
+        const double sig = r[R_FRAC], isig = r[R_ISIG];
+        const double k2c = kclamp ? copysign(K2CLAMP, k2) : k2;
+        const double s2c = kclamp ? (1.0 - K2CLAMP * K2CLAMP) : s2;
+        const double rs = rsq64(s2c);
+        const double sq = s2c * rs;
+        const double frac = sq * sig;
+        const double ifrac = rs * isig;
+        double da = -0.5 * fma(Kz, frac, k3 * ifrac);
+        double db = 0.5 * fma(Kx * Ky, frac, -k1 * k2c * ifrac);
+        {
+            // the mirror-zero form (see beam_bin_mixed)
+            const double dk1 = -qx * ik, dk2 = -qy * ik;
+            const double dk3 = fma(RR[6], ux, fma(RR[7], uy, fma(RR[8], uz, Kz)));
+            const double u2 = dk2 * (k2 + Ky);
+            const double w2 = fma(Kx, dk2, dk1 * k2);
+            const double dan = 0.5 * rs * fma(Kz * sig, u2, -isig * dk3);
+            const double dbn = -0.5 * rs * fma(Kx * Ky * sig, u2, isig * w2);
+            const bool mir = !kclamp && s2 > 0.5;
+            da = mir ? dan : da;
+            db = mir ? dbn : db;
+        }
+        const double m1 = -k1 * k2c * rs, m2 = sq, m3 = -k2c * k3 * rs;
+        const double r13 = kclamp ? rsq64(s2) : rs;
+        const double ev1 = -k3 * r13, ev3 = k1 * r13;
+        double hx = uy, hy = -ux;
+        if (fwd) {
+            hx = bi.yt - r[R_COM + 1];
+            hy = -(bi.xt - r[R_COM]);
+        }
+        const double rh = rsq64(fma(hx, hx, hy * hy));
+        hx *= rh;
+        hy *= rh;
+        const double h1 = fma(RR[0], hx, RR[1] * hy);
+        const double h2 = fma(RR[3], hx, RR[4] * hy);
+        const double h3 = fma(RR[6], hx, RR[7] * hy);
+        const double p = fma(h1, m1, fma(h2, m2, h3 * m3));
+        const double qh = fma(h1, ev1, h3 * ev3);
+        gg = fma(p, da, qh * db);
+        hh = fma(p, db, -qh * da);
+    }
+    // f (gg I + hh J) A = w P A + w' Q A, w = f (gg + i hh), w' = f (gg - i hh)
+    const double ta = FR * gg, tb = FI * hh, tc = FR * hh, td = FI * gg;
+    const double wr = ta - tb, wi = tc + td, vr_ = ta + tb, vi = td - tc;
+    const double* rp = r + R_RP64;
+    const double* rm = r + R_RM64;
+    acc[0] = fma(wr, rp[0], fma(-wi, rp[1], acc[0]));
+    acc[1] = fma(wr, rp[1], fma(wi, rp[0], acc[1]));
+    acc[2] = fma(wr, rp[2], fma(-wi, rp[3], acc[2]));
+    acc[3] = fma(wr, rp[3], fma(wi, rp[2], acc[3]));
+    acc[4] = fma(vr_, rm[0], fma(-vi, rm[1], acc[4]));
+    acc[5] = fma(vr_, rm[1], fma(vi, rm[0], acc[5]));
+    acc[6] = fma(vr_, rm[2], fma(-vi, rm[3], acc[6]));
+    acc[7] = fma(vr_, rm[3], fma(vi, rm[2], acc[7]));
 }
 
 // Beams per FP32 partial sum of the mixed kernels: every MIXED_GROUP consecutive beams

@@ -194,11 +194,19 @@ __host__ __device__ inline void make_beam_record(const double* ampl, const doubl
     rg[G_RP + 2] = (float)(0.5 * (A[4] + A[7])); rg[G_RP + 3] = (float)(0.5 * (A[5] - A[6]));
     rg[G_RM + 0] = (float)(0.5 * (A[0] - A[3])); rg[G_RM + 1] = (float)(0.5 * (A[1] + A[2]));
     rg[G_RM + 2] = (float)(0.5 * (A[4] - A[7])); rg[G_RM + 3] = (float)(0.5 * (A[5] + A[6]));
-    // Karczewski coefficients over krev (the mixed body's krev-scaled polarisation)
+    rec[R_RP64 + 0] = 0.5 * (A[0] + A[3]); rec[R_RP64 + 1] = 0.5 * (A[1] - A[2]);
+    rec[R_RP64 + 2] = 0.5 * (A[4] + A[7]); rec[R_RP64 + 3] = 0.5 * (A[5] - A[6]);
+    rec[R_RM64 + 0] = 0.5 * (A[0] - A[3]); rec[R_RM64 + 1] = 0.5 * (A[1] + A[2]);
+    rec[R_RM64 + 2] = 0.5 * (A[4] - A[7]); rec[R_RM64 + 3] = 0.5 * (A[5] + A[6]);
+    // Karczewski coefficients over krev (the krev-scaled polarisation of the restructured bodies)
     const double isig = sqrt(rec[R_KY1]);
-    rg[G_C1] = (float)(-0.5 * prop2[2] * rec[R_FRAC] / krev);
-    rg[G_C2] = (float)(-0.5 * isig / krev);
-    rg[G_C3] = (float)(0.5 * prop2[0] * prop2[1] * rec[R_FRAC] / krev);
+    rec[R_C64 + 0] = -0.5 * prop2[2] * rec[R_FRAC] / krev;
+    rec[R_C64 + 1] = -0.5 * isig / krev;
+    rec[R_C64 + 2] = 0.5 * prop2[0] * prop2[1] * rec[R_FRAC] / krev;
+    rec[R_ISIG] = isig;
+    rg[G_C1] = (float)rec[R_C64 + 0];
+    rg[G_C2] = (float)rec[R_C64 + 1];
+    rg[G_C3] = (float)rec[R_C64 + 2];
     float* rf = reinterpret_cast<float*>(rec + R_F32);
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) rf[F_RR + 3 * i + j] = (float)rr.a[i][j];
@@ -211,6 +219,9 @@ __host__ __device__ inline void make_beam_record(const double* ampl, const doubl
     const double tpa = 6.283185307179586 * amax;
     rf[F_QA2] = (float)(tpa * tpa);
     rf[F_NV] = (float)nv;
+    double c2 = 0.0;
+    for (int q = 0; q < 8; q++) c2 += A[q] * A[q];
+    rf[F_C2] = (float)c2;
     for (int j = 0; j < 3 && j < nv; j++) {
         rf[F_EDX + j] = (float)rec[R_EDX + j];
         rf[F_EDY + j] = (float)rec[R_EDY + j];

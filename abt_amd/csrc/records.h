@@ -41,7 +41,12 @@ enum : int {
     R_G32 = 76,     // second FP32 block (float index G_* below), 6 doubles
     R_F32 = 82,     // FP32 block (float index F_* below), 12 doubles
     R_SIGMA = 94,   // -1 when the amplitude matrix was sign-flipped (anti-parallel), else +1
-    REC = 96        // padded to a multiple of 4 doubles (32 bytes)
+    // FP64 copies of the mixed kernel's per-beam constants, for the restructured FP64 body
+    R_RP64 = 96,    // [4] r+ = (A(1,:) - i A(2,:)) / 2: re/im of columns 1, 2
+    R_RM64 = 100,   // [4] r- = (A(1,:) + i A(2,:)) / 2
+    R_C64 = 104,    // [3] the Karczewski coefficients G_C1, G_C2, G_C3 in FP64
+    R_ISIG = 107,   // sqrt(1 - prop2_y^2)
+    REC = 108       // padded to a multiple of 4 doubles (32 bytes)
 };
 
 // float offsets inside the FP32 block of a beam record
@@ -55,7 +60,8 @@ enum : int {
     F_NV = 16,      // number of vertices
     F_EDX = 17,     // [3] edge x_{j+1} - x_j
     F_EDY = 20,     // [3] edge y_{j+1} - y_j
-    F_END = 23
+    F_C2 = 23,      // squared Frobenius norm of the amplitude matrix
+    F_END = 24
 };
 
 // float offsets inside the second FP32 block
@@ -81,6 +87,7 @@ struct DevGrid {
     double krev_k2clamp;                    // krev * K2CLAMP (mixed kernel's k_y clamp test)
     float inv_lambda_f;                     // 1/lambda (FP32 body of the mixed kernel)
     float inv_krev_f;                       // 1/krev
+    double inv_krev;                        // 1/krev (restructured FP64 body)
     const double* x;                        // [nbins] far-field bin positions
     const double* y;
     const double* z;
@@ -90,7 +97,13 @@ struct DevGrid {
     const double* kr_rev;                   // [nbins] k|x|/(2 pi)
     const double* theta;                    // [ntheta] theta_vals, rad
     const double* phi;                      // [nphi] phi_vals, rad
+    const double* etab;                     // [ETAB_DOUBLES] e^{i pi j / ETAB_N} (restructured FP64 body)
 };
+
+// Table of the restructured FP64 body's exponentials: cos and sin of pi j / ETAB_N for
+// j = -ETAB_N .. ETAB_N, interleaved, built on the host with the C library.
+constexpr int ETAB_N = 512;
+constexpr int ETAB_DOUBLES = 2 * (2 * ETAB_N + 1);
 
 // REAL*4 literals of the reference, widened to double
 constexpr double PI_R4 = 3.1415927410125732;
@@ -98,5 +111,18 @@ constexpr double K2CLAMP = 0.9999989867210388;
 constexpr double KMIN = 9.999999974752427e-07;
 constexpr double FWD_DOT = 0.9990000128746033;
 constexpr double ANTI_THR = 9.999999747378752e-05;
+
+// The wavelength-derived constants of a grid: waveno = 2 pi_r4 / lambda (diff_mod.f90:332,
+// with the REAL*4 pi) and the scaled forms the kernel bodies read.
+inline void devgrid_constants(DevGrid& g, double lambda)
+{
+    g.lambda = lambda;
+    g.waveno = 2.0 * PI_R4 / lambda;
+    g.krev = g.waveno / 6.283185307179586476925;
+    g.krev_k2clamp = g.krev * K2CLAMP;
+    g.inv_lambda_f = (float)(1.0 / lambda);
+    g.inv_krev_f = (float)(1.0 / g.krev);
+    g.inv_krev = 1.0 / g.krev;
+}
 
 }  // namespace abt

@@ -13,12 +13,14 @@
 #include <string.h>
 
 #include <algorithm>
+#include <list>
 #include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
 
 #include "../../include/abt_diff.h"
+#include "ffbody.h"
 #include "frame.h"
 #include "records.h"
 #include "rt_internal.h"
@@ -229,9 +231,10 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     int rc = ensure_device();
     if (rc) return rc;
     const int64_t nb = nth * nph;
-    std::vector<double> h((size_t)(5 * nb + 3 * nph + nth));
+    std::vector<double> h((size_t)(5 * nb + 3 * nph + nth + abt::ETAB_DOUBLES));
     double *x = h.data(), *y = x + nb, *z = y + nb, *rad = z + nb, *krr = rad + nb;
-    double *cp = krr + nb, *sp = cp + nph, *phv = sp + nph, *thv = phv + nph;
+    double *cp = krr + nb, *sp = cp + nph, *phv = sp + nph, *thv = phv + nph, *etab = thv + nth;
+    abt::fill_exp_table(etab);
     memcpy(phv, ph, sizeof(double) * nph);
     memcpy(thv, th, sizeof(double) * nth);
     if (xs) {
@@ -264,12 +267,7 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     g->d.ntheta = (int32_t)nth;
     g->d.nbins = (int32_t)nb;
     g->d.theta_i = theta_i_of(nth, th);
-    g->d.lambda = lambda;
-    g->d.waveno = waveno;
-    g->d.krev = krev;
-    g->d.krev_k2clamp = krev * abt::K2CLAMP;
-    g->d.inv_lambda_f = (float)(1.0 / lambda);
-    g->d.inv_krev_f = (float)(1.0 / krev);
+    abt::devgrid_constants(g->d, lambda);
     g->d.x = d;
     g->d.y = d + nb;
     g->d.z = d + 2 * nb;
@@ -279,10 +277,128 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     g->d.sphi = d + 5 * nb + nph;
     g->d.phi = d + 5 * nb + 2 * nph;
     g->d.theta = d + 5 * nb + 3 * nph;
+    g->d.etab = d + 5 * nb + 3 * nph + nth;
     g->nbins = nb;
     *out = g;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Grid cache of the host entry points.  PBT calls diff_main once per orientation and,
+// in -memory mode, diffraction once per beam from several OpenMP threads, always on the
+// same bins: the device grid is built on the first call and kept, keyed by the exact
+// contents of its inputs (angles, wavelength and, for diffraction, the caller's bin
+// positions), so later calls neither rebuild nor allocate it.  Entries in use are never
+// evicted; when every slot is busy a call builds a private grid and frees it.
+namespace {
+
+struct GridEntry {
+    int64_t nth = 0, nph = 0;
+    double lambda = 0.0;
+    std::vector<double> th, ph, xyz;   // xyz empty: bins from make_far_field_bins
+    abt_dev_grid* g = nullptr;
+    int users = 0;
+    uint64_t last = 0;
+};
+
+constexpr size_t GRID_CACHE_SLOTS = 8;
+std::mutex g_grid_mu;
+std::list<GridEntry> g_grids;
+uint64_t g_grid_clock = 0;
+
+bool same(const std::vector<double>& v, const double* p, int64_t n)
+{
+    return (int64_t)v.size() == n && (n == 0 || memcmp(v.data(), p, sizeof(double) * (size_t)n) == 0);
+}
+
+// A cached grid for these inputs (xs, ys, zs may be null), with a use held: release it
+// with grid_release.  *priv is set when the grid is private to the call (cache full).
+int grid_acquire(int64_t nth, const double* th, int64_t nph, const double* ph, const double* xs,
+                 const double* ys, const double* zs, double lambda, abt_dev_grid** out, bool* priv)
+{
+    const int64_t nb = nth * nph;
+    *priv = false;
+    {
+        std::lock_guard<std::mutex> lk(g_grid_mu);
+        for (GridEntry& e : g_grids) {
+            if (e.nth != nth || e.nph != nph || e.lambda != lambda) continue;
+            if (!same(e.th, th, nth) || !same(e.ph, ph, nph)) continue;
+            if (xs ? (e.xyz.size() != (size_t)(3 * nb) || memcmp(e.xyz.data(), xs, sizeof(double) * nb) != 0 ||
+                      memcmp(e.xyz.data() + nb, ys, sizeof(double) * nb) != 0 ||
+                      memcmp(e.xyz.data() + 2 * nb, zs, sizeof(double) * nb) != 0)
+                   : !e.xyz.empty())
+                continue;
+            e.users++;
+            e.last = ++g_grid_clock;
+            *out = e.g;
+            return 0;
+        }
+    }
+    abt_dev_grid* g = nullptr;
+    int rc = grid_build(nth, th, nph, ph, xs, ys, zs, lambda, &g);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    if (g_grids.size() >= GRID_CACHE_SLOTS) {
+        // evict the least recently used idle entry
+        auto victim = g_grids.end();
+        for (auto it = g_grids.begin(); it != g_grids.end(); ++it)
+            if (it->users == 0 && (victim == g_grids.end() || it->last < victim->last)) victim = it;
+        if (victim == g_grids.end()) {
+            *priv = true;
+            *out = g;
+            return 0;
+        }
+        abt_grid_destroy(victim->g);
+        g_grids.erase(victim);
+    }
+    GridEntry e;
+    e.nth = nth;
+    e.nph = nph;
+    e.lambda = lambda;
+    e.th.assign(th, th + nth);
+    e.ph.assign(ph, ph + nph);
+    if (xs) {
+        e.xyz.resize((size_t)(3 * nb));
+        memcpy(e.xyz.data(), xs, sizeof(double) * nb);
+        memcpy(e.xyz.data() + nb, ys, sizeof(double) * nb);
+        memcpy(e.xyz.data() + 2 * nb, zs, sizeof(double) * nb);
+    }
+    e.g = g;
+    e.users = 1;
+    e.last = ++g_grid_clock;
+    g_grids.push_back(std::move(e));
+    *out = g;
+    return 0;
+}
+
+void grid_release(abt_dev_grid* g, bool priv)
+{
+    if (priv) {
+        abt_grid_destroy(g);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    for (GridEntry& e : g_grids)
+        if (e.g == g) {
+            e.users--;
+            return;
+        }
+}
+
+void grid_cache_clear()
+{
+    std::lock_guard<std::mutex> lk(g_grid_mu);
+    for (auto it = g_grids.begin(); it != g_grids.end();) {
+        if (it->users == 0) {
+            abt_grid_destroy(it->g);
+            it = g_grids.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // host preparation of beam records
@@ -387,6 +503,7 @@ int abt_init(int device)
 
 int abt_finalize(void)
 {
+    grid_cache_clear();
     std::lock_guard<std::mutex> lk(g_reg_mu);
     for (ThreadState* ts : g_threads) {
         for (auto& b : ts->bufs) {
@@ -471,7 +588,8 @@ int abt_far_field_dev(const abt_dev_grid* g, const double* dev_recs, const doubl
                       size_t work_bytes, void* stream)
 {
     if (!g || !dev_out) return fail(ABT_ERR_ARG, "abt_far_field_dev: null argument");
-    if (precision != ABT_FP64 && precision != ABT_MIXED) return fail(ABT_ERR_ARG, "unknown precision %d", precision);
+    if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_FAST)
+        return fail(ABT_ERR_ARG, "unknown precision %d", precision);
     if (nb < 0) return fail(ABT_ERR_ARG, "negative beam count");
     int rc = ensure_device();
     if (rc) return rc;
@@ -522,13 +640,17 @@ int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff, const a
                   const abt_beams* ext, int precision, double* ampl_far_beam, double* ampl_far_ext_diff)
 {
     if (!grid || !ampl_far_beam || !ampl_far_ext_diff) return fail(ABT_ERR_ARG, "abt_diff_main: null argument");
-    if (precision != ABT_FP64 && precision != ABT_MIXED) return fail(ABT_ERR_ARG, "unknown precision %d", precision);
+    if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_FAST)
+        return fail(ABT_ERR_ARG, "unknown precision %d", precision);
     int rc = check_beams_host(beams, "abt_diff_main(beams)");
     if (rc) return rc;
     rc = check_beams_host(ext, "abt_diff_main(ext)");
     if (rc) return rc;
+    if (!grid->theta || !grid->phi) return fail(ABT_ERR_ARG, "abt_diff_main: null angles");
     abt_dev_grid* g = nullptr;
-    rc = abt_grid_create(grid, lambda, &g);
+    bool priv = false;
+    rc = grid_acquire(grid->n_theta, grid->theta, grid->n_phi, grid->phi, nullptr, nullptr, nullptr, lambda, &g,
+                      &priv);
     if (rc) return rc;
     ThreadState& ts = tstate();
     rc = stream_of(ts);
@@ -556,7 +678,7 @@ int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff, const a
             if (e != hipSuccess) rc = fail(ABT_ERR_HIP, "copy-back: %s", hipGetErrorString(e));
         }
     }
-    abt_grid_destroy(g);
+    grid_release(g, priv);
     return rc;
 }
 
@@ -569,9 +691,11 @@ int abt_diffraction(int64_t n_phi, int64_t n_theta, const double* xfar, const do
     if (!xfar || !yfar || !zfar || !phi_vals || !theta_vals || !ampl || !vk7 || !prop || !verts || !amplC)
         return fail(ABT_ERR_ARG, "abt_diffraction: null argument");
     if (nv < 3) return fail(ABT_ERR_ARG, "abt_diffraction: aperture needs at least 3 vertices");
-    if (precision != ABT_FP64 && precision != ABT_MIXED) return fail(ABT_ERR_ARG, "unknown precision %d", precision);
+    if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_FAST)
+        return fail(ABT_ERR_ARG, "unknown precision %d", precision);
     abt_dev_grid* g = nullptr;
-    int rc = grid_build(n_theta, theta_vals, n_phi, phi_vals, xfar, yfar, zfar, lambda, &g);
+    bool priv = false;
+    int rc = grid_acquire(n_theta, theta_vals, n_phi, phi_vals, xfar, yfar, zfar, lambda, &g, &priv);
     if (rc) return rc;
     ThreadState& ts = tstate();
     rc = stream_of(ts);
@@ -587,7 +711,7 @@ int abt_diffraction(int64_t n_phi, int64_t n_theta, const double* xfar, const do
             if (e != hipSuccess) rc = fail(ABT_ERR_HIP, "copy-back: %s", hipGetErrorString(e));
         }
     }
-    abt_grid_destroy(g);
+    grid_release(g, priv);
     return rc;
 }
 
@@ -631,9 +755,12 @@ int abt_finalise(const abt_grid* grid, double lambda, double* ampl_far_beam, dou
                  double* mueller_beam, double* mueller_ext, double* mueller_1d,
                  double* mueller_1d_beam, double* mueller_1d_ext, double* params)
 {
-    if (!grid || !ampl_far_beam || !ampl_far_ext_diff) return fail(ABT_ERR_ARG, "abt_finalise: null argument");
+    if (!grid || !ampl_far_beam || !ampl_far_ext_diff || !grid->theta || !grid->phi)
+        return fail(ABT_ERR_ARG, "abt_finalise: null argument");
     abt_dev_grid* g = nullptr;
-    int rc = abt_grid_create(grid, lambda, &g);
+    bool priv = false;
+    int rc = grid_acquire(grid->n_theta, grid->theta, grid->n_phi, grid->phi, nullptr, nullptr, nullptr, lambda, &g,
+                          &priv);
     if (rc) return rc;
     ThreadState& ts = tstate();
     rc = stream_of(ts);
@@ -668,7 +795,7 @@ int abt_finalise(const abt_grid* grid, double lambda, double* ampl_far_beam, dou
             if (e != hipSuccess) rc = fail(ABT_ERR_HIP, "copy-back: %s", hipGetErrorString(e));
         }
     }
-    abt_grid_destroy(g);
+    grid_release(g, priv);
     return rc;
 }
 

@@ -20,8 +20,12 @@ LIB_PATH = os.path.join(HERE, "libabt_diff.so")
 ABT_FP64 = 0
 ABT_MIXED = 1
 ABT_FP64_ACCURATE = 2
+ABT_FP64_FAST = 3
+ABT_MIXED_GATED = 4
 PRECISIONS = {"fp64": ABT_FP64, "mixed": ABT_MIXED, "fp64_accurate": ABT_FP64_ACCURATE,
-              ABT_FP64: ABT_FP64, ABT_MIXED: ABT_MIXED, ABT_FP64_ACCURATE: ABT_FP64_ACCURATE}
+              "fp64_fast": ABT_FP64_FAST, "mixed_gated": ABT_MIXED_GATED,
+              ABT_FP64: ABT_FP64, ABT_MIXED: ABT_MIXED, ABT_FP64_ACCURATE: ABT_FP64_ACCURATE,
+              ABT_FP64_FAST: ABT_FP64_FAST, ABT_MIXED_GATED: ABT_MIXED_GATED}
 PI_R4 = float(np.float32(3.14159265358979))
 
 # every entry point declared in include/abt_diff.h
@@ -33,7 +37,8 @@ EXPORTS = [
     "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
     "abt_far_field_incoherent_dev", "abt_batch_plan_create", "abt_batch_plan_destroy",
     "abt_batch_workspace", "abt_batch_set_tiles", "abt_batch_get_tiles", "abt_batch_autotune",
-    "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_adjoint_plan_create",
+    "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_batch_gate_stats",
+    "abt_adjoint_plan_create",
     "abt_adjoint_plan_destroy", "abt_adjoint_run", "abt_find_vis", "abt_find_vis_workspace",
     "abt_find_vis_dev", "abt_vis_rotations", "abt_vis_rotate_dev",
 ]
@@ -107,6 +112,7 @@ def load():
     L.abt_batch_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.abt_batch_capture.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.abt_batch_launch.argtypes = [vp, vp]
+    L.abt_batch_gate_stats.argtypes = [vp, ctypes.POINTER(i32), vp]
     L.abt_adjoint_plan_create.argtypes = [vp, i32, vp, vp, i, ctypes.POINTER(vp)]
     L.abt_adjoint_plan_destroy.argtypes = [vp]
     L.abt_adjoint_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -365,6 +371,12 @@ class BatchPlan:
     def launch(self, stream=None):
         """Replay the captured graph."""
         _check(load().abt_batch_launch(self.handle, _stream_ptr(stream)))
+
+    def refined_bins(self, stream=None):
+        """mixed_gated plans: bins the last run recomputed in FP64 (synchronises)."""
+        n = ctypes.c_int32()
+        _check(load().abt_batch_gate_stats(self.handle, ctypes.byref(n), _stream_ptr(stream)))
+        return n.value
 
     def close(self):
         if self.handle is not None and self.handle.value:

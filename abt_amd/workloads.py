@@ -9,7 +9,8 @@ Beam-set generators:
   * `random_triangles`    -- generic seeded triangular apertures (tests)
   * `stress_beams`        -- the seeded 200,000-aperture diffraction stress
                              test (planar triangles normal to their propagation
-                             vector, centroids in a 100 um cube)
+                             vector, centroids in a 100 um cube, edge directions
+                             uniform in the aperture plane)
   * `column_beams`        -- beam sets shaped like the tracer's output for a
                              hexagonal column (ext-diff beams on illuminated
                              facets plus reflected / refracted outgoing beams
@@ -109,19 +110,24 @@ def reflected_facets(n, seed=0, box=40.0, size=(1.0, 4.0)) -> BeamSet:
     return BeamSet.from_triangles(amp, vk7, prop, np.full(n, np.pi / 2), tri)
 
 
-def stress_beams(n=200_000, seed=20240601) -> BeamSet:
+def stress_beams(n=200_000, seed=20240601, min_angle=0.0) -> BeamSet:
     """Diffraction stress test: n planar triangles, centroids uniform in a 100 um cube,
     two edge vectors of length U(0.3, 5) um in the plane normal to the propagation
-    vector (uniform on the sphere) at an angle in [0.3, pi - 0.3] rad to each other,
-    amplitude entries (N(0,1) + i N(0,1)) * 10^U(-6,0), e-perp a seeded unit vector
-    normal to prop.  Generator: numpy PCG64 with `seed`."""
+    vector (uniform on the sphere), each with a uniform direction in that plane (so
+    sliver triangles occur as often as the stated distribution makes them), amplitude
+    entries (N(0,1) + i N(0,1)) * 10^U(-6,0), e-perp a seeded unit vector normal to prop.
+    min_angle > 0 instead keeps the angle between the two edges in
+    [min_angle, pi - min_angle] (no slivers; the earlier form of this generator, which
+    some tests use for its particular apertures).  Generator: numpy PCG64 with `seed`."""
     rng = np.random.default_rng(seed)
     c = rng.uniform(0.0, 100.0, (n, 3))
     prop = _unit(rng.normal(size=(n, 3)))
     a, b = _perp_basis(prop, rng)
     ang1 = rng.uniform(0, 2 * np.pi, (n, 1))
-    # angle between the two edges in [0.3, pi - 0.3] rad either way round: no sliver triangles
-    ang2 = ang1 + rng.uniform(0.3, np.pi - 0.3, (n, 1)) * rng.choice([-1.0, 1.0], (n, 1))
+    if min_angle > 0:
+        ang2 = ang1 + rng.uniform(min_angle, np.pi - min_angle, (n, 1)) * rng.choice([-1.0, 1.0], (n, 1))
+    else:
+        ang2 = rng.uniform(0, 2 * np.pi, (n, 1))
     l1 = rng.uniform(0.3, 5.0, (n, 1))
     l2 = rng.uniform(0.3, 5.0, (n, 1))
     e1 = l1 * (np.cos(ang1) * a + np.sin(ang1) * b)

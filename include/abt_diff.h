@@ -42,9 +42,19 @@ enum abt_status {
 enum abt_precision {
     ABT_FP64 = 0,         /* plain FP64 kernel, the reference's formulas */
     ABT_MIXED = 1,        /* FP32 body, FP64 phases, FP64 series near the forward limit */
-    ABT_FP64_ACCURATE = 2 /* batched plans only: ABT_FP64 with the far-field phase k|x - com|
+    ABT_FP64_ACCURATE = 2,/* batched plans only: ABT_FP64 with the far-field phase k|x - com|
                              formed without cancellation (k(|x - com| - |x|) plus the per-bin
                              k|x|); for finite-difference gradient checks */
+    ABT_FP64_FAST = 3,    /* FP64 throughout in the mixed kernel's restructured algorithm (three
+                             vertex exponentials per beam and bin instead of the split sum's twelve
+                             trigonometric pairs); agrees with ABT_FP64 to ~1e-12 of the largest
+                             amplitude */
+    ABT_MIXED_GATED = 4   /* batched plans only: ABT_MIXED, plus a per-bin bound of the FP32 body's
+                             error over the orientation sum; every bin whose bound exceeds the
+                             accuracy gate (|dS11| <= 1e-6 S11, |dSij| <= 1e-5 max(|Sij|, 1e-3 S11))
+                             is recomputed over all orientations with ABT_FP64_FAST.  The
+                             per-orientation integrals and amplitudes keep the ABT_MIXED values
+                             outside the recomputed bins */
 };
 
 /* One tree of outgoing beams (the flattened outbeamtype records of
@@ -227,6 +237,9 @@ int abt_batch_capture(abt_batch_plan* plan, const double* dev_recs_beam,
                       const double* dev_vrecs_ext, double* dev_mueller, double* dev_mueller_1d,
                       double* dev_integrals, double* dev_ampl, void* stream);
 int abt_batch_launch(const abt_batch_plan* plan, void* stream);
+/* ABT_MIXED_GATED plans: the number of bins the last run recomputed in FP64 (0 for other
+ * precisions).  Synchronises the stream. */
+int abt_batch_gate_stats(const abt_batch_plan* plan, int32_t* refined_bins, void* stream);
 
 /* ---- adjoint: gradients for shape and roughness retrieval ---------------------
  * No reference counterpart (the reference has no derivatives); the forward it

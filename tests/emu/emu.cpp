@@ -17,12 +17,10 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
     DevGrid g;
     memset(&g, 0, sizeof(g));
     g.nphi = nphi; g.ntheta = ntheta; g.nbins = nphi * ntheta; g.theta_i = theta_i;
-    g.lambda = lambda;
-    g.waveno = 2.0 * PI_R4 / lambda;
-    g.krev = g.waveno / 6.283185307179586476925;
-    g.krev_k2clamp = g.krev * K2CLAMP;
-    g.inv_lambda_f = (float)(1.0 / lambda);
-    g.inv_krev_f = (float)(1.0 / g.krev);
+    devgrid_constants(g, lambda);
+    static double etab[ETAB_DOUBLES];
+    fill_exp_table(etab);
+    g.etab = etab;
     const int nbins = g.nbins;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int bin = 0; bin < nbins; bin++) {
@@ -35,6 +33,8 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
         for (int64_t b = 0; b < nb; b++) {
             if (prec == 0)
                 beam_bin_fp64(recs + b * REC, vrecs, g, bi, cphi[iphi], sphi[iphi], fast_diff, acc);
+            else if (prec == 4)     // the restructured FP64 body
+                beam_bin_fp64r(recs + b * REC, vrecs, g, bi, g.etab, fast_diff, acc);
             else if (prec == 3) {   // the FP64 model (adjbody.h)
                 AdjBeam B;
                 adj_beam_from_record(recs + b * REC, B);
@@ -49,7 +49,7 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
                     }
             }
         }
-        if (prec == 1) mixed_finish(acc, cphi[iphi], sphi[iphi]);
+        if (prec == 1 || prec == 4) mixed_finish(acc, cphi[iphi], sphi[iphi]);
         if (prec == 3) model_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
         for (int q = 0; q < 4; q++) {
             out[2 * (bin + (int64_t)nbins * q)] = acc[2 * q];

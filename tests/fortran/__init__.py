@@ -1,6 +1,7 @@
 """Runs the Fortran drop-in (tests/fortran/_build/abt_driver, linking
-abt_amd/fortran/diff_mod.f90 and the reference's types_mod) on a beam set and
-reads its outputs back.  Test infrastructure."""
+abt_amd/fortran/diff_mod.f90 compiled against the stand-in tests/fortran/types_mod.f90)
+on a beam set and reads its outputs back.  Test infrastructure; the drivers are built
+by __graft_entry__.build() (make -C tests/fortran)."""
 import os
 import subprocess
 import tempfile
