@@ -19,6 +19,10 @@ no pickle) and `to_json`/`from_json`, a text form whose records carry the
 matrix written row by row as [[[re, im], [re, im]], [[re, im], [re, im]]], the
 order of the reference's own beam export (beam_loop_mod.f90:254-263).  Floats
 are written with Python's shortest round-trip repr, so a JSON round trip is exact.
+
+`outbeams_from_pbt_json` reads the reference's own export, the near-field beam tree that
+`-export_beam` writes (beam.json, beam_loop_mod.f90:136-340), and forms the outgoing beams
+the way add_to_outbeam_tree does (beam_loop_mod.f90:2745-2790).
 """
 from __future__ import annotations
 
@@ -26,6 +30,8 @@ import dataclasses
 import json
 
 import numpy as np
+
+PI_R4 = float(np.float32(3.14159265358979))
 
 
 @dataclasses.dataclass
@@ -139,3 +145,72 @@ def concat(sets) -> BeamSet:
     return BeamSet(np.concatenate([s.ampl for s in sets]), np.concatenate([s.vk7 for s in sets]),
                    np.concatenate([s.prop for s in sets]), np.concatenate([s.fov for s in sets]),
                    np.concatenate(off), np.concatenate([s.verts for s in sets]))
+
+
+def outbeams_from_pbt_json(path, verts, faces, lam, with_ids=False):
+    """The outgoing beams of a PBT beam.json (`-export_beam`, beam_loop_mod.f90:136-340).
+
+    The file holds the near-field beam tree: per beam its propagation vector `prop`,
+    `is_int`, and `field_in` (per illuminating facet: `fi`, the amplitude matrix row by
+    row, `e_perp`).  add_to_outbeam_tree (beam_loop_mod.f90:2745-2790) makes one outgoing
+    beam per field_in facet of an external beam that reaches the far field:
+        ampl = field_in%ampl, prop_out = beam%prop, vk7 = field_in%e_perp,
+        aperture = the facet fi (vertices from the particle geometry),
+        fov = 8 lambda / d, d = 2 sqrt(A / pi), A the summed area of the beam's field_in facets.
+    The file does not record which facets reach the far field (the `is_outgoing` flag that
+    recursion_ext clears when an external beam meets the particle again,
+    beam_loop_mod.f90:884), so every field_in facet of every external beam (is_int false) is
+    taken: the outgoing set of a particle whose external beams all escape (convex
+    particles), a superset otherwise.  Internal beams are skipped.
+
+    verts (nv, 3) and faces (nf, k) 0-based: the particle geometry the tree was traced on
+    (geometry%v and geometry%f(:)%vi), facet fi of the file is faces[fi - 1].  Returns a
+    BeamSet (and, with_ids, the 1-based facet ids and the file's beam ids)."""
+    with open(path) as f:
+        doc = json.load(f)
+    tree = doc["beam tree"]
+    V = np.asarray(verts, dtype=np.float64)
+    F = [np.asarray(fc, dtype=np.int64) for fc in faces]
+
+    def area(fi):
+        p = V[F[fi - 1]]
+        n = np.zeros(3)
+        for j in range(len(p)):   # Newell's formula, any planar polygon
+            n += np.cross(p[j], p[(j + 1) % len(p)])
+        return 0.5 * np.linalg.norm(n)
+
+    ampl, vk7, prop, fov, vo, vs, fis, ids = [], [], [], [], [0], [], [], []
+    for beam in tree:
+        if beam["is_int"]:
+            continue
+        fields = beam["field_in"]
+        if not fields:
+            continue
+        a_tot = 0.0
+        for fd in fields:
+            a_tot = a_tot + area(int(fd["fi"]))
+        fv = (lam / (2.0 * np.sqrt(a_tot / PI_R4))) * 8.0     # the REAL*4 pi of misc_submod.f90:12
+        for fd in fields:
+            fi = int(fd["fi"])
+            rows = fd["ampl"]          # [[a11, a12], [a21, a22]], each [re, im]
+            a = np.empty(8)
+            for r in range(2):
+                for c in range(2):
+                    a[2 * (r + 2 * c):2 * (r + 2 * c) + 2] = rows[r][c]
+            ampl.append(a)
+            vk7.append(fd["e_perp"])
+            prop.append(beam["prop"])
+            fov.append(fv)
+            p = V[F[fi - 1]]
+            vs.append(p)
+            vo.append(vo[-1] + len(p))
+            fis.append(fi)
+            ids.append(int(beam["beam id"]))
+    if not ampl:
+        out = BeamSet.empty()
+    else:
+        out = BeamSet(np.array(ampl), np.array(vk7, dtype=np.float64), np.array(prop, dtype=np.float64),
+                      np.array(fov), np.array(vo, dtype=np.int64), np.concatenate(vs))
+    if with_ids:
+        return out, np.array(fis, dtype=np.int64), np.array(ids, dtype=np.int64)
+    return out

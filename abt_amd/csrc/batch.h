@@ -23,6 +23,8 @@ struct BatchArgs {
     double* ipart;            // [norient][ntiles][6] partial integrals
     unsigned int* counter;    // work queue
     double* ampl_out;         // [norient][8][nbins] k-scaled total amplitude per orientation, or null
+    double* store;            // [norient][16][nbins] k-scaled beam then ext amplitude per orientation
+                              // (the -scaling path), or null
     // gated mixed precision (the error scale of the FP32 pass and the bins it leaves to FP64)
     double* epart;            // [nchunks][nbins] partial sums of S11_o * I_o (I_o: incoherent beam sum)
     const int32_t* list;      // [nbins] bins recomputed in FP64 (list mode), ascending
@@ -45,6 +47,16 @@ extern "C" int abt_launch_batch(const abt::BatchArgs* a, int prec, int tb, int b
 // workgroups, partial sums left in the argument block's buffers (the autotuner's timing).
 extern "C" int abt_launch_batch_pass(const abt::BatchArgs* a, int prec, int tb, int bpt, int mode, int wgs,
                                      hipStream_t st);
+// -scaling (outputs_mod.f90:146-184): the persistent pass storing every orientation's beam and
+// ext amplitudes (a->store), the unscaled integrals reduced into raw[norient][6], the divisors
+// sqrt(scatt_beam / beam_energy[o]) and sqrt(scatt_ext / ext_energy[o]) into div[norient][2],
+// the Mueller sums and all six integrals recomputed from the divided amplitudes (one lane per
+// bin, orientations of a chunk in order), then the chunk and tile reductions and the phi
+// integral.  a->ampl_out (non-null) receives the divided totals.
+extern "C" int abt_launch_batch_scaled(const abt::BatchArgs* a, int prec, int tb, int bpt, int wgs, int norient,
+                                       const double* beam_energy, const double* ext_energy, double* raw,
+                                       double* div, double* mueller, double* mueller_1d, double* integrals,
+                                       hipStream_t st);
 // Gated mixed precision: the mixed pass with its error scale (into est[nbins]), the bins whose
 // error bound exceeds the gate listed (list, *count), those bins recomputed in FP64 by the
 // restructured body (`wgs2` persistent workgroups of the 16 x 1 list kernel) and written over

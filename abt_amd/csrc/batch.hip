@@ -159,7 +159,7 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
             bi[j].x = a.g.x[bc]; bi[j].y = a.g.y[bc]; bi[j].z = a.g.z[bc];
             const int bt = iphi + a.g.nphi * a.g.theta_i;
             bi[j].xt = a.g.x[bt]; bi[j].yt = a.g.y[bt]; bi[j].zt = a.g.z[bt];
-            bi[j].rad = PREC == 3 ? a.g.rad[bc] : 0.0;
+            bi[j].rad = (PREC == 3 || PREC == 4) ? a.g.rad[bc] : 0.0;
             cp[j] = PROJ ? 0.0 : a.g.cphi[iphi];   // the projector bodies rotate after the sum
             sp[j] = PROJ ? 0.0 : a.g.sphi[iphi];
         }
@@ -187,7 +187,8 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                 for (int j = 0; j < BPT; j++) {
                     if (PROJ) {
                         const int iphi = bins[j] % a.g.nphi;
-                        mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
+                        if (PREC == 4) fp64r_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi], a.g.kr_rev[bins[j]]);
+                        else mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                     }
                     if (PREC == 3) model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bins[j]]);
 #pragma unroll
@@ -202,7 +203,8 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                 const int bc = bins[j];
                 if (PROJ) {
                     const int iphi = bc % a.g.nphi;
-                    mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
+                    if (PREC == 4) fp64r_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi], a.g.kr_rev[bc]);
+                    else mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                 }
                 if (PREC == 3) model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bc]);
                 double xb[8], xe[8], xt[8], m[16];
@@ -214,6 +216,14 @@ __global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
                 }
                 mueller16(xt, m);
                 if (slot < nslots) {
+                    if (a.store) {
+                        double* so = a.store + (int64_t)o * 16 * nbins + bc;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) {
+                            so[(int64_t)q * nbins] = xb[q];
+                            so[(int64_t)(8 + q) * nbins] = xe[q];
+                        }
+                    }
                     if (a.ampl_out) {
                         double* ao = a.ampl_out + (int64_t)o * 8 * nbins + bc;
 #pragma unroll
@@ -299,6 +309,90 @@ __global__ __launch_bounds__(BB) void k_batch_phi(const double* __restrict__ M, 
 }
 
 __global__ void k_zero_u32(unsigned int* p) { if (threadIdx.x == 0 && blockIdx.x == 0) p[0] = 0u; }
+
+// ---- -scaling (outputs_mod.f90:146-184) ----------------------------------------
+//
+// Per orientation the reference divides the beam amplitudes by sqrt(scatt_beam /
+// beam_energy_out) and the ext amplitudes by sqrt(scatt_ext_diff / ext_energy_out), scatt_*
+// being that orientation's own cross sections, then remakes the Mueller matrices and the
+// integrated parameters.  The cross sections need every bin of the orientation, so the
+// persistent pass stores the amplitudes, and this pass divides and sums them.
+
+// div[o] = {sqrt(scatt_beam / E_beam), sqrt(scatt_ext / E_ext)} from the unscaled integrals
+__global__ __launch_bounds__(BB) void k_scale_div(const double* __restrict__ raw, const double* __restrict__ eb,
+                                                  const double* __restrict__ ee, int norient,
+                                                  double* __restrict__ div)
+{
+    const int o = blockIdx.x * BB + threadIdx.x;
+    if (o >= norient) return;
+    div[2 * o] = sqrt(raw[6 * o + 2] / eb[o]);
+    div[2 * o + 1] = sqrt(raw[6 * o + 4] / ee[o]);
+}
+
+// grid (ntiles of BB bins, nchunks): the chunk's orientations in order for one bin per lane;
+// mpart[c] and ipart[o][t] (ntiles = ceil(nbins / BB)) as the persistent kernel writes them.
+__global__ __launch_bounds__(BB) void k_batch_scaled(BatchArgs a, const double* __restrict__ div, int ntiles)
+{
+    __shared__ double red[BB / 64][6];
+    const int tid = threadIdx.x;
+    const int nbins = a.g.nbins;
+    const int t = blockIdx.x, c = blockIdx.y;
+    const int bc = t * BB + tid;
+    const bool live = bc < nbins;
+    const int b = live ? bc : nbins - 1;
+    const double w1 = a.w1[b], w2 = a.w2[b];
+    double msum[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) msum[e] = 0.0;
+    const int o0 = a.chunk_o[c], o1 = a.chunk_o[c + 1];
+    for (int o = o0; o < o1; o++) {
+        const double db = div[2 * o], de = div[2 * o + 1];
+        const double* so = a.store + (int64_t)o * 16 * nbins + b;
+        double xb[8], xe[8], xt[8], m[16];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            xb[q] = so[(int64_t)q * nbins] / db;          // outputs_mod.f90:151
+            xe[q] = so[(int64_t)(8 + q) * nbins] / de;    // outputs_mod.f90:154
+            xt[q] = xb[q] + xe[q];                        // outputs_mod.f90:156
+        }
+        mueller16(xt, m);
+#pragma unroll
+        for (int e = 0; e < 16; e++) msum[e] += m[e];
+        if (live && a.ampl_out) {
+            double* ao = a.ampl_out + (int64_t)o * 8 * nbins + bc;
+#pragma unroll
+            for (int q = 0; q < 8; q++) ao[(int64_t)q * nbins] = xt[q];
+        }
+        double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        if (live) {
+            const double sb = mueller_s11(xb), se = mueller_s11(xe);
+            v[0] = w1 * m[0];
+            v[1] = w2 * m[0];
+            v[2] = w1 * sb;
+            v[3] = w2 * sb;
+            v[4] = w1 * se;
+            v[5] = w2 * se;
+        }
+#pragma unroll
+        for (int q = 0; q < 6; q++) v[q] = wave_sum(v[q]);
+        if ((tid & 63) == 0) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) red[tid >> 6][q] = v[q];
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double* ip = a.ipart + ((int64_t)o * ntiles + t) * 6;
+#pragma unroll
+            for (int q = 0; q < 6; q++) ip[q] = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+        }
+        __syncthreads();
+    }
+    if (live && o1 > o0) {
+        double* mo = a.mpart + (int64_t)c * 16 * nbins + bc;
+#pragma unroll
+        for (int e = 0; e < 16; e++) mo[(int64_t)e * nbins] = msum[e];
+    }
+}
 
 // ---- gated mixed precision ---------------------------------------------------
 //
@@ -424,6 +518,36 @@ extern "C" int abt_launch_batch(const BatchArgs* a, int prec, int tb, int bpt, i
                        (const double*)a->mpart, a->nchunks, n, mueller);
     hipLaunchKernelGGL(k_batch_reduce_i, dim3((unsigned)((norient * 6 + BB - 1) / BB)), dim3(BB), 0, st,
                        (const double*)a->ipart, a->ntiles, norient, integrals);
+    if (mueller_1d)
+        hipLaunchKernelGGL(k_batch_phi, dim3((unsigned)((16 * a->g.ntheta + BB - 1) / BB)), dim3(BB), 0, st,
+                           (const double*)mueller, a->g.phi, a->g.nphi, a->g.ntheta, mueller_1d);
+    return (int)hipGetLastError();
+}
+
+extern "C" int abt_launch_batch_scaled(const BatchArgs* a, int prec, int tb, int bpt, int wgs, int norient,
+                                       const double* beam_energy, const double* ext_energy, double* raw,
+                                       double* div, double* mueller, double* mueller_1d, double* integrals,
+                                       hipStream_t st)
+{
+    using namespace abt;
+    const void* kern = abt_batch_kernel(prec, tb, bpt, BMODE_PLAIN);
+    if (!kern || !a->store) return (int)hipErrorInvalidValue;
+    BatchArgs s = *a;
+    s.ampl_out = nullptr;   // the divided totals are written by the scaling pass
+    int e = launch_pass(kern, s, wgs, st);
+    if (e) return e;
+    hipLaunchKernelGGL(k_batch_reduce_i, dim3((unsigned)((norient * 6 + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->ipart, a->ntiles, norient, raw);
+    hipLaunchKernelGGL(k_scale_div, dim3((unsigned)((norient + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)raw, beam_energy, ext_energy, norient, div);
+    const int nt1 = (a->g.nbins + BB - 1) / BB;
+    hipLaunchKernelGGL(k_batch_scaled, dim3((unsigned)nt1, (unsigned)a->nchunks), dim3(BB), 0, st, *a,
+                       (const double*)div, nt1);
+    const int64_t n = (int64_t)16 * a->g.nbins;
+    hipLaunchKernelGGL(k_batch_reduce_m, dim3((unsigned)((n + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->mpart, a->nchunks, n, mueller);
+    hipLaunchKernelGGL(k_batch_reduce_i, dim3((unsigned)((norient * 6 + BB - 1) / BB)), dim3(BB), 0, st,
+                       (const double*)a->ipart, nt1, norient, integrals);
     if (mueller_1d)
         hipLaunchKernelGGL(k_batch_phi, dim3((unsigned)((16 * a->g.ntheta + BB - 1) / BB)), dim3(BB), 0, st,
                            (const double*)mueller, a->g.phi, a->g.nphi, a->g.ntheta, mueller_1d);

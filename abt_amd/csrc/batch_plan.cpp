@@ -110,6 +110,13 @@ struct abt_batch_plan {
     int32_t* d_list = nullptr;       // [nbins]
     int32_t* d_count = nullptr;
     int wgs_list = 0;                // persistent workgroups of the FP64 list pass
+    // -scaling: per-orientation amplitudes, unscaled integrals, divisors; the caller's energies
+    void* smem = nullptr;
+    double* d_store = nullptr;       // [norient][16][nbins]
+    double* d_raw = nullptr;         // [norient][6]
+    double* d_div = nullptr;         // [norient][2]
+    const double* beam_energy = nullptr;
+    const double* ext_energy = nullptr;
     int max_chunks = 1;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -151,6 +158,7 @@ void fill_args(const abt_batch_plan* p, const Shape& s, const int32_t* chunk, co
     a->ipart = p->d_ipart;
     a->counter = p->d_counter;
     a->ampl_out = nullptr;
+    a->store = p->d_store;
     a->epart = p->d_epart;
     a->list = p->d_list;
     a->list_count = p->d_count;
@@ -314,7 +322,42 @@ int abt_batch_plan_destroy(abt_batch_plan* p)
     if (!p) return 0;
     drop_graph(p);
     if (p->mem) (void)hipFree(p->mem);
+    if (p->smem) (void)hipFree(p->smem);
     delete p;
+    return 0;
+}
+
+int abt_batch_set_scaling(abt_batch_plan* p, const double* beam_energy_out, const double* ext_energy_out)
+{
+    if (!p) return abt_fail(ABT_ERR_ARG, "abt_batch_set_scaling: null plan");
+    if (!beam_energy_out != !ext_energy_out)
+        return abt_fail(ABT_ERR_ARG, "abt_batch_set_scaling: give both energy arrays, or neither");
+    if (beam_energy_out && p->prec == ABT_MIXED_GATED)
+        return abt_fail(ABT_ERR_ARG, "abt_batch_set_scaling: not available for the gated mixed plan");
+    int rc = abt_ensure_device(nullptr);
+    if (rc) return rc;
+    drop_graph(p);   // a captured graph holds the other launch sequence
+    if (!beam_energy_out) {
+        if (p->smem) (void)hipFree(p->smem);
+        p->smem = nullptr;
+        p->d_store = p->d_raw = p->d_div = nullptr;
+        p->beam_energy = p->ext_energy = nullptr;
+        return 0;
+    }
+    if (!p->smem) {
+        const size_t nstore = (size_t)16 * p->g->nbins * p->norient;
+        const size_t bytes = sizeof(double) * (nstore + 8 * (size_t)p->norient);
+        void* m = nullptr;
+        hipError_t e = hipMalloc(&m, bytes);
+        if (e != hipSuccess)
+            return abt_fail(ABT_ERR_HIP, "abt_batch_set_scaling: %zu bytes: %s", bytes, hipGetErrorString(e));
+        p->smem = m;
+        p->d_store = (double*)m;
+        p->d_raw = p->d_store + nstore;
+        p->d_div = p->d_raw + 6 * (size_t)p->norient;
+    }
+    p->beam_energy = beam_energy_out;
+    p->ext_energy = ext_energy_out;
     return 0;
 }
 
@@ -378,6 +421,7 @@ int abt_batch_autotune(abt_batch_plan* p, const double* recs_b, const double* vr
             if (rc) break;
             abt::BatchArgs a;
             fill_args(p, s, p->d_chunk_tune, recs_b, vrecs_b, recs_e, vrecs_e, &a);
+            a.store = nullptr;   // the timing is the far-field pass's
             float ms = 0.f;
             for (int rep = 0; rep < 2 && !rc; rep++) {   // first launch warms up
                 he = hipEventRecord(e0, st);
@@ -427,7 +471,11 @@ int abt_batch_run(const abt_batch_plan* p, const double* recs_b, const double* v
     fill_args(p, p->shape, p->d_chunk, recs_b, vrecs_b, recs_e, vrecs_e, &a);
     a.ampl_out = dev_ampl;
     int e;
-    if (p->prec == ABT_MIXED_GATED)
+    if (p->beam_energy)
+        e = abt_launch_batch_scaled(&a, kernel_prec(p->prec), p->shape.tb, p->shape.bpt, p->shape.wgs, p->norient,
+                                    p->beam_energy, p->ext_energy, p->d_raw, p->d_div, dev_mueller,
+                                    dev_mueller_1d, dev_integrals, (hipStream_t)stream);
+    else if (p->prec == ABT_MIXED_GATED)
         e = abt_launch_batch_gated(&a, p->shape.tb, p->shape.bpt, p->shape.wgs, p->wgs_list, p->norient,
                                    p->d_est, p->d_flags, p->d_list, p->d_count, dev_mueller, dev_mueller_1d,
                                    dev_integrals, (hipStream_t)stream);
@@ -484,6 +532,7 @@ int abt_batch_workspace(const abt_batch_plan* p, size_t* bytes)
                                2 * (size_t)nbins);
     if (p->prec == ABT_MIXED_GATED)
         *bytes += sizeof(double) * ((size_t)nbins * p->max_chunks + nbins) + sizeof(int32_t) * 2 * (size_t)nbins;
+    if (p->smem) *bytes += sizeof(double) * ((size_t)16 * nbins * p->norient + 8 * (size_t)p->norient);
     return 0;
 }
 

@@ -68,6 +68,7 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
     bi.x = g.x[bc]; bi.y = g.y[bc]; bi.z = g.z[bc];
     const int bt = iphi + g.nphi * g.theta_i;
     bi.xt = g.x[bt]; bi.yt = g.y[bt]; bi.zt = g.z[bt];
+    bi.rad = PREC == 4 ? g.rad[bc] : 0.0;
     const double cp = g.cphi[iphi], sp = g.sphi[iphi];
     double acc[8];
 #pragma unroll
@@ -135,7 +136,8 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
         }
     }
     if (!active) return;
-    if (PREC == 1 || PREC == 4) mixed_finish(acc, cp, sp);
+    if (PREC == 1) mixed_finish(acc, cp, sp);
+    if (PREC == 4) fp64r_finish(acc, cp, sp, g.kr_rev[bin]);
     const int nbins = g.nbins;
     if (PREC == 2) {   // incoherent intensity: one double per bin, never chunked
         out[bin] = acc[0];

@@ -601,37 +601,64 @@ __host__ __device__ __forceinline__ void expi_pi64(double t, const double* __res
     *s = fma(s0, cr, c0 * sr);
 }
 
-// one edge of the rotation-invariant form factor in FP64 (see edge_exp): c_j sinc(pi d) w_j w_{j+1}
-__host__ __device__ __forceinline__ void edge_exp64(double wr0, double wi0, double wr1, double wi1, double d,
-                                                    double qx, double qy, double edx, double edy, double& ssr,
-                                                    double& ssi)
+// e^{2 pi i t} for an FP64 t (revolutions), from the same table: t = m + j / (2 ETAB_N) + r,
+// |r| <= 1 / (4 ETAB_N), e^{2 pi i t} = tab[j] e^{2 pi i r}
+__host__ __device__ __forceinline__ void expi_2pi64(double t, const double* __restrict__ tab, double* c, double* s)
 {
-    const double Er = fma(wr0, wr1, -wi0 * wi1);
-    const double Ei = fma(wr0, wi1, wi0 * wr1);
-    const double sx = fma(wi1, wr0, -wr1 * wi0);   // sin(pi d)
-    const double pd = PI64 * d;
-    const double z = pd * pd;
-    // sin(x)/x for |x| < pi/64 (truncation < 1e-16); sin(pi d) / (pi d) elsewhere, whose
-    // relative error is below 4e-15 there
-    const double sp = fma(z, fma(z, fma(z, -1.0 / 5040.0, 1.0 / 120.0), -1.0 / 6.0), 1.0);
-    const double S = fabs(d) < 1.0 / 64.0 ? sp : sx * rcp64_1(pd);
-    const double t = fma(qx, edy, -qy * edx) * S;
-    ssr = fma(t, Er, ssr);
-    ssi = fma(t, Ei, ssi);
+    const double m = rint(t);
+    const double u = t - m;                                       // [-1/2, 1/2]
+    const double jf = rint((double)(2 * ETAB_N) * u);
+    const double r = fma(-1.0 / (double)(2 * ETAB_N), jf, u);     // |r| <= 1 / (4 ETAB_N)
+    const int j = (int)jf + ETAB_N;
+    const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
+    const double y = (2.0 * PI64) * r;
+    const double z = y * y;
+    const double sr = fma(y * z, fma(z, 1.0 / 120.0, -1.0 / 6.0), y);
+    const double cr = fma(z, fma(z, 1.0 / 24.0, -0.5), 1.0);
+    *c = fma(c0, cr, -s0 * sr);
+    *s = fma(s0, cr, c0 * sr);
+}
+
+// Difference form of one edge of the rotation-invariant form factor, from the full vertex
+// phase exponentials W = e^{2 pi i t} of its two ends (t in revolutions):
+//   c_j sinc(pi d) e^{i pi (t_j + t_{j+1})} = c_j (W_{j+1} - W_j) / (2 pi i d),  d = t_{j+1} - t_j,
+// c_j = qx dy_j - qy dx_j.  Below |d| = 1e-3 (relative error of the difference there ~2e-16 /
+// (pi |d|)) the product form with the Taylor polynomials of sinc(pi d) and e^{i pi d}, which
+// only edges nearly normal to q reach.
+__host__ __device__ __forceinline__ void edge_diff64(double Wr0, double Wi0, double Wr1, double Wi1, double d,
+                                                     double qx, double qy, double edx, double edy, double& ssr,
+                                                     double& ssi)
+{
+    const double c = fma(qx, edy, -qy * edx);
+    if (fabs(d) >= 1e-3) {
+        const double k = c * rcp64_1((2.0 * PI64) * d);
+        ssr = fma(k, Wi1 - Wi0, ssr);       // c (W1 - W0) / (2 pi i d)
+        ssi = fma(-k, Wr1 - Wr0, ssi);
+    } else {
+        const double x = PI64 * d, z = x * x;
+        const double sinc = fma(z, fma(z, 1.0 / 120.0, -1.0 / 6.0), 1.0);
+        const double ch = fma(z, fma(z, 1.0 / 24.0, -0.5), 1.0);          // e^{i pi d}
+        const double sh = x * fma(z, fma(z, 1.0 / 120.0, -1.0 / 6.0), 1.0);
+        const double k = c * sinc;
+        ssr = fma(k, fma(Wr0, ch, -Wi0 * sh), ssr);
+        ssi = fma(k, fma(Wr0, sh, Wi0 * ch), ssi);
+    }
 }
 
 // Restructured FP64 body: one beam, one bin; adds to the projector sums S+ (acc[0..3]) and
-// S- (acc[4..7]) that mixed_finish turns into the amplitude matrix.
+// S- (acc[4..7]) that fp64r_finish turns into the amplitude matrix.  bi.rad = |x| is read.
 __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict__ r,
                                                         const double* __restrict__ vrecs,
                                                         const DevGrid& g, const BinIn& bi,
                                                         const double* __restrict__ etab,
                                                         int fast_diff, double acc[8])
 {
-    // ---- geometry: bin direction from the aperture centre, |r - com| ----
+    // ---- geometry: bin direction from the aperture centre; the far-field phase relative to
+    // the bin's own, k (|x - c| - |x|) = k (|c|^2 - 2 x.c) / (|x - c| + |x|), formed without
+    // cancellation (~1e-13 rad; the per-bin e^{i k |x|} is applied after the beam sum) ----
     const double dx = bi.x - r[R_COM], dy = bi.y - r[R_COM + 1], dz = bi.z - r[R_COM + 2];
     const double s = fma(dx, dx, fma(dy, dy, dz * dz));
-    const double invd = rsq64(s);
+    const double invd = rsq64_1(s);
     const double ux = dx * invd, uy = dy * invd, uz = dz * invd;
     if (fast_diff) {
         if (ux * r[R_PROP0] + uy * r[R_PROP0 + 1] + uz * r[R_PROP0 + 2] < r[R_COSFOV]) return;
@@ -640,7 +667,8 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
     const double qx = fma(-K[0], ux, fma(-K[1], uy, fma(-K[2], uz, r[R_KP2])));
     const double qy = fma(-K[3], ux, fma(-K[4], uy, fma(-K[5], uz, r[R_KP2 + 1])));
     const double k1r = r[R_KP2] - qx, k2r = r[R_KP2 + 1] - qy;
-    const double kd = g.krev * (s * invd);
+    const double rc = fma(bi.x, r[R_COM], fma(bi.y, r[R_COM + 1], bi.z * r[R_COM + 2]));
+    const double kd = g.krev * (fma(-2.0, rc, r[R_CC]) * rcp64_1(fma(s, invd, bi.rad)));   // revolutions
     const double q2 = fma(qx, qx, qy * qy);
     const double lam = g.lambda;
 
@@ -685,34 +713,34 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
     } else {
         double ssr = 0.0, ssi = 0.0;
         if (tri) {
-            double t[3], wr[3], wi[3];
+            double t[3], Wr[3], Wi[3];
 #pragma unroll
             for (int l = 0; l < 3; l++) {
                 t[l] = fma(qx, r[R_EX + l], fma(qy, r[R_EY + l], kd));
-                expi_pi64(t[l], etab, &wr[l], &wi[l]);
+                expi_2pi64(t[l], etab, &Wr[l], &Wi[l]);
             }
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const int jn = j == 2 ? 0 : j + 1;
-                edge_exp64(wr[j], wi[j], wr[jn], wi[jn], t[jn] - t[j], qx, qy, r[R_EDX + j], r[R_EDY + j], ssr, ssi);
+                edge_diff64(Wr[j], Wi[j], Wr[jn], Wi[jn], t[jn] - t[j], qx, qy, r[R_EDX + j], r[R_EDY + j], ssr, ssi);
             }
         } else {
             const int nv = (int)r[R_NV];
             const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
             const double t0 = fma(qx, vr[V_X], fma(qy, vr[V_Y], kd));
-            double wr0, wi0;
-            expi_pi64(t0, etab, &wr0, &wi0);
-            double tp = t0, wrp = wr0, wip = wi0;
+            double Wr0, Wi0;
+            expi_2pi64(t0, etab, &Wr0, &Wi0);
+            double tp = t0, Wrp = Wr0, Wip = Wi0;
             for (int j = 0; j < nv; j++) {
-                double tn = t0, wrn = wr0, win = wi0;
+                double tn = t0, Wrn = Wr0, Win = Wi0;
                 if (j < nv - 1) {
                     tn = fma(qx, vr[VREC * (j + 1) + V_X], fma(qy, vr[VREC * (j + 1) + V_Y], kd));
-                    expi_pi64(tn, etab, &wrn, &win);
+                    expi_2pi64(tn, etab, &Wrn, &Win);
                 }
-                edge_exp64(wrp, wip, wrn, win, tn - tp, qx, qy, vr[VREC * j + V_DX], vr[VREC * j + V_DY], ssr, ssi);
+                edge_diff64(Wrp, Wip, Wrn, Win, tn - tp, qx, qy, vr[VREC * j + V_DX], vr[VREC * j + V_DY], ssr, ssi);
                 tp = tn;
-                wrp = wrn;
-                wip = win;
+                Wrp = Wrn;
+                Wip = Win;
             }
         }
         const double sc = -rcp64_1(6.283185307179586 * lam * q2);
@@ -733,7 +761,7 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
     if (!kclamp && !fwd) {
         const double P0 = fma(K[3], uy, -K[4] * ux);
         const double Q0 = fma(-uz, k2r, K[5]);
-        const double rh = rsq64_1(fma(ux, ux, uy * uy)) * rcp64_1(s2);
+        const double rh = rsq64_1(fma(ux, ux, uy * uy) * (s2 * s2));   // 1 / (|(ux, uy)| s2)
         const double P = P0 * rh, Q = Q0 * rh;
         const double A = fma(C[0], s2, C[1] * k3);
         const double B = fma(C[2], s2, C[1] * (k1 * k2));
@@ -845,6 +873,14 @@ __host__ __device__ __forceinline__ void apply_temp_rot1(double acc[8], double c
             o[4 + 2 * rr + ri] = fma(-cp, x1, -sp * x2);
         }
     for (int q = 0; q < 8; q++) acc[q] = o[q];
+}
+
+// After the beam sum of beam_bin_fp64r: the projector sums to the amplitude matrix (with
+// temp_rot1), then the bin's far-field phase e^{i k |x|} (kr_rev = k |x| / 2 pi).
+__host__ __device__ __forceinline__ void fp64r_finish(double acc[8], double cp, double sp, double kr_rev)
+{
+    mixed_finish(acc, cp, sp);
+    apply_bin_phase(acc, kr_rev);
 }
 
 }  // namespace abt

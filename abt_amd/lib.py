@@ -35,9 +35,10 @@ EXPORTS = [
     "abt_vertex_record_doubles", "abt_prepare_beams_host", "abt_prepare_beams_dev",
     "abt_grid_create", "abt_grid_destroy", "abt_grid_bins", "abt_far_field_workspace",
     "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
-    "abt_far_field_incoherent_dev", "abt_batch_plan_create", "abt_batch_plan_destroy",
+    "abt_far_field_incoherent_dev",
+    "abt_batch_plan_create", "abt_batch_plan_destroy",
     "abt_batch_workspace", "abt_batch_set_tiles", "abt_batch_get_tiles", "abt_batch_autotune",
-    "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_batch_gate_stats",
+    "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_batch_gate_stats", "abt_batch_set_scaling",
     "abt_adjoint_plan_create",
     "abt_adjoint_plan_destroy", "abt_adjoint_run", "abt_find_vis", "abt_find_vis_workspace",
     "abt_find_vis_dev", "abt_vis_rotations", "abt_vis_rotate_dev",
@@ -113,6 +114,7 @@ def load():
     L.abt_batch_capture.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]
     L.abt_batch_launch.argtypes = [vp, vp]
     L.abt_batch_gate_stats.argtypes = [vp, ctypes.POINTER(i32), vp]
+    L.abt_batch_set_scaling.argtypes = [vp, vp, vp]
     L.abt_adjoint_plan_create.argtypes = [vp, i32, vp, vp, i, ctypes.POINTER(vp)]
     L.abt_adjoint_plan_destroy.argtypes = [vp]
     L.abt_adjoint_run.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -371,6 +373,23 @@ class BatchPlan:
     def launch(self, stream=None):
         """Replay the captured graph."""
         _check(load().abt_batch_launch(self.handle, _stream_ptr(stream)))
+
+    def set_scaling(self, beam_energy_out=None, ext_energy_out=None):
+        """-scaling (outputs_mod.f90:146-184): float64 device tensors of norient energies,
+        kept referenced by the plan and read by every run; None, None turns it off."""
+        if (beam_energy_out is None) != (ext_energy_out is None):
+            raise ValueError("give both energy arrays, or neither")
+        if beam_energy_out is not None:
+            for t in (beam_energy_out, ext_energy_out):
+                if not t.is_cuda or str(t.dtype) != "torch.float64" or t.numel() != self.norient \
+                        or not t.is_contiguous():
+                    raise ValueError("energies: contiguous float64 device tensors of norient values")
+            self._energies = (beam_energy_out, ext_energy_out)
+            _check(load().abt_batch_set_scaling(self.handle, beam_energy_out.data_ptr(),
+                                                 ext_energy_out.data_ptr()))
+        else:
+            _check(load().abt_batch_set_scaling(self.handle, None, None))
+            self._energies = None
 
     def refined_bins(self, stream=None):
         """mixed_gated plans: bins the last run recomputed in FP64 (synchronises)."""

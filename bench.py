@@ -1,21 +1,28 @@
 """Benchmark: beam.bin diffractions/s of the far-field diffraction path on one MI355X.
 
-Workload (default): the seeded diffraction stress test -- 200,000 planar
-triangular apertures (centroids in a 100 um cube, edges 0.3-5 um normal to a
-random propagation vector, amplitudes (N+iN) 10^U(-6,0)), wavelength 0.532 um,
-grid 1801 theta (0.1 deg) x 360 phi (1 deg) = 648,360 bins.
+Headline workload: the batched orientation average -- 1000 orientations (uniform on
+SO(3), seeded) of a Gaussian-rough hexagonal column (r 10 um, l 20 um, sigma 0.5 um,
+2112 facets, m 1.31, 10 internal reflections; 4.3M beams + 1.06M external-diffraction
+beams), 721 theta x 180 phi = 129,780 bins, lambda 0.532 um.  One step = one replay of
+the plan's HIP graph: every orientation's beam tree and external-diffraction tree
+diffracted onto every bin, each orientation's Mueller matrix summed into the
+orientation sum, its phi integral and the per-orientation cross sections and asymmetry
+parameters.  `value` = (beams + ext beams) * bins / (seconds per step).
 
-One step = one pass of the hot path over the whole beam set with the raw beam
-arrays already resident in HBM: per-beam frame set-up (device), the coherent
-far-field sum of every beam onto every bin (mixed-precision kernel), and the
-Mueller epilogue (total / beam / ext Mueller matrices, phi integration, theta
-integrals).  `value` = beams * bins / (seconds per step).
+The headline plan is the one that meets the accuracy gate -- every element of the
+orientation-summed Mueller matrix within |dS_ij| <= 1e-5 max(|S_ij|, 1e-3 S11) and S11
+within 1e-6 of the FP64 reference -- checked in the same run against the plain FP64
+drop-in kernel (the reference's formulas) run as the reference drivers run it (per
+orientation diff_main + finalise, summed).  Records are prepared on the host with the
+reference's operation order and libm (bit-identical to the oracle's frames) and are
+resident in HBM before the timed region.
 
-Also measured in the same run: the plain FP64 kernel on the same inputs (its
-rate and the mixed/FP64 ratio), the accuracy of the mixed result against the
-FP64 one per Mueller element, the dominant kernel's roofline fraction (HIP
-events on the launch stream), and a CPU baseline (the FP64 C oracle, OpenMP,
-on a bounded sample of the same workload).
+Sections in the same JSON line: the mixed-precision kernel (FP32 body) and the gated
+mixed plan on the same workload; the 200,000-aperture stress test (one orientation,
+1801 x 360 bins); the large absorbing column (a stated number of orientations); the
+adjoint; the reference-facing host entry points per call on the reference's CPU case;
+facet visibility; CPU baselines (the C restatement of the reference, OpenMP, on bounded
+samples).
 """
 import argparse
 import json
@@ -30,157 +37,14 @@ sys.path.insert(0, ROOT)
 
 F_ALG = 250.0            # algorithmic FLOP per beam.bin (triangular aperture)
 PEAK = {"mixed": 157.3e12, "fp64": 78.6e12}   # MI355X vector peaks, FLOP/s
+PEAK_OF = {"mixed": "mixed", "mixed_gated": "mixed", "fp64": "fp64", "fp64_fast": "fp64"}
 METRIC = "beam·bin diffractions/s mixed-prec vs FP64 at ≤1e-5 Mueller rel err"
+LAM = 0.532
+GATE = {"S11": 1e-6, "elem": 1e-5, "floor": 1e-3}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--beams", type=int, default=200_000)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
-    ap.add_argument("--fp64-steps", type=int, default=1, help="FP64 comparison steps (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=96, help="apertures in the CPU baseline sample (0: skip)")
-    ap.add_argument("--orient", type=int, default=1000,
-                    help="orientations of the batched orientation-average workload (0: skip)")
-    ap.add_argument("--orient-fp64", type=int, default=1,
-                    help="run the FP64 plain and FP64 batched comparisons of the orientation average (0: skip)")
-    ap.add_argument("--adjoint-orient", type=int, default=64,
-                    help="orientations of the adjoint workload (0: skip)")
-    ap.add_argument("--large-orient", type=int, default=0,
-                    help="orientations of the large absorbing column (20,736 facets; 0: skip)")
-    ap.add_argument("--vis-beams", type=int, default=4096,
-                    help="beams of the facet-visibility workload (0: skip)")
-    ap.add_argument("--batch-tiles", default=None,
-                    help="fixed tile shape of the orientation-batched kernel, e.g. 16x1 (default: autotune)")
-    ap.add_argument("--json-out", default=None)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "path is a design (orientation/beam sharding + one RCCL reduce), not built.",
-              file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-    from abt_amd import lib
-    from abt_amd.workloads import GRIDS, grid, stress_beams
-
-    lam = 0.532
-    dev = torch.device("cuda:0")
-    th, ph = grid(*GRIDS["stress"])
-    t0 = time.time()
-    beams = stress_beams(args.beams)
-    log(f"workload: {beams.n} apertures, grid {len(th)} x {len(ph)} ({time.time() - t0:.1f} s to generate)")
-    g = lib.DeviceGrid(th, ph, lam)
-    g.waveno = 2 * lib.PI_R4 / lam
-    nb, nbins = beams.n, g.nbins
-    raw = {k: torch.from_numpy(getattr(beams, k)).to(dev) for k in
-           ("ampl", "vk7", "prop", "fov", "vert_off", "verts")}
-    R = lib.load().abt_record_doubles()
-    recs = torch.empty((nb, R), dtype=torch.float64, device=dev)
-    amp = torch.empty(8 * nbins, dtype=torch.float64, device=dev)
-    amp_ext = torch.empty(8 * nbins, dtype=torch.float64, device=dev)
-    mueller = torch.empty(48 * nbins, dtype=torch.float64, device=dev)
-    m1d = torch.empty(48 * len(th), dtype=torch.float64, device=dev)
-    params = torch.empty(16, dtype=torch.float64, device=dev)
-    ws = lib.far_field_workspace(g, nb, args.precision)
-    work = torch.empty(max(ws, 8) // 8 + 1, dtype=torch.float64, device=dev)
-    fws = lib.load().abt_finalise_workspace(g.handle)
-    fwork = torch.empty(fws // 8 + 1, dtype=torch.float64, device=dev)
-    stream = torch.cuda.current_stream()
-    k_ev = []
-
-    def step(prec, time_kernel=False):
-        lib.prepare_beams_dev(raw, nb, lam, recs, fov_value=None, stream=stream)
-        if time_kernel:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(stream)
-        lib.far_field_dev(g, recs, nb, amp, precision=prec, work=work, stream=stream)
-        if time_kernel:
-            e1.record(stream)
-            k_ev.append((e0, e1))
-        amp_ext.zero_()
-        lib.finalise_dev(g, amp, amp_ext, mueller, m1d, params, work=fwork, stream=stream)
-
-    def timed(prec, steps, warmup):
-        k_ev.clear()
-        for _ in range(warmup):
-            step(prec)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for _ in range(steps):
-            step(prec, time_kernel=True)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t) / steps
-        kms = float(np.mean([a.elapsed_time(b) for a, b in k_ev]))
-        return dt, kms
-
-    dt, kms = timed(args.precision, args.steps, args.warmup)
-    value = nb * nbins / dt
-    log(f"{args.precision}: {dt * 1e3:.1f} ms/step, kernel {kms:.1f} ms, {value:.4e} beam.bin/s")
-    res_main = amp.clone()
-    m_main = mueller[:16 * nbins].clone()
-
-    out = {
-        "metric": METRIC, "value": value, "unit": "beam·bin/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32+f64" if args.precision == "mixed" else "f64", "data": "synthetic",
-        "config": {"workload": "diffraction stress test: 200,000 seeded triangular apertures, "
-                               "1801 theta x 360 phi bins, lambda 0.532 um",
-                   "beams": nb, "bins": nbins, "beam_bins": nb * nbins,
-                   "precision": args.precision, "step": "device frame set-up + far-field sum + Mueller epilogue"},
-    }
-    ach = F_ALG * nb * nbins / (kms * 1e-3)
-    kname = "k_far_field<%d,16>" % (1 if args.precision == "mixed" else 0)
-    traffic = _traffic(kname) if nb == 200_000 else None
-    out["roofline"] = {"bound": "valu", "achieved": ach / 1e12, "peak": PEAK[args.precision] / 1e12,
-                       "unit": "TFLOP/s", "frac": ach / PEAK[args.precision], "traffic": traffic,
-                       "kernel": kname,
-                       "kernel_ms": kms, "flop_per_beam_bin": F_ALG}
-
-    if args.fp64_steps > 0 and args.precision == "mixed":
-        dt64, kms64 = timed("fp64", args.fp64_steps, 0)
-        v64 = nb * nbins / dt64
-        log(f"fp64: {dt64 * 1e3:.1f} ms/step, kernel {kms64:.1f} ms, {v64:.4e} beam.bin/s")
-        a64 = amp.cpu().numpy()
-        m64 = mueller[:16 * nbins].cpu().numpy().reshape(16, nbins)
-        am = res_main.cpu().numpy()
-        mm = m_main.cpu().numpy().reshape(16, nbins)
-        out["fp64"] = {"value": v64, "ms_per_step": dt64 * 1e3, "kernel_ms": kms64,
-                       "roofline_frac": F_ALG * nb * nbins / (kms64 * 1e-3) / PEAK["fp64"]}
-        out["speedup_vs_fp64"] = value / v64
-        inc = torch.empty(nbins, dtype=torch.float64, device=dev)
-        lib.far_field_incoherent_dev(g, recs, nb, inc, stream=stream)
-        torch.cuda.synchronize()
-        # the epilogue scaled the amplitudes by k in place; the incoherent sum is unscaled
-        out["accuracy"] = accuracy(am, a64, mm, m64, nbins, inc.cpu().numpy() * g.waveno ** 2)
-
-    if args.cpu_sample > 0:
-        out["cpu_baseline"] = cpu_baseline(beams, th, ph, lam, args.cpu_sample)
-    del recs, amp, amp_ext, mueller, work, raw
-    torch.cuda.empty_cache()
-    if args.orient > 0:
-        out["orientation_average"] = orientation_average(args.orient, args.steps, lam, dev, bool(args.orient_fp64),
-                                                         args.batch_tiles)
-    if args.adjoint_orient > 0:
-        out["adjoint"] = adjoint_bench(args.adjoint_orient, lam, dev)
-    if args.large_orient > 0:
-        out["large_column"] = orientation_average(args.large_orient, args.steps, lam, dev, True, None,
-                                                  particle="large_column", grid_name="large", with_kappa=True)
-    if args.vis_beams > 0:
-        out["visibility"] = visibility_bench(args.vis_beams, max(args.steps, 20), max(args.warmup, 2), dev,
-                                             args.cpu_sample > 0)
-    line = json.dumps(out)
-    if args.json_out:
-        with open(args.json_out, "w") as f:
-            f.write(line + "\n")
-    print(line, flush=True)
 
 
 def _traffic(kname):
@@ -192,239 +56,399 @@ def _traffic(kname):
     return json.load(open(tpath)).get(kname, {}).get("bytes_per_launch")
 
 
-def accuracy(am, a64, mm, m64, nbins, incoherent):
-    """Mixed vs FP64 on the full workload, per Mueller element.
-    S11_rel: |dS11| / S11;  norm_elem: |dS_ij| / S11 (normalised element);
-    rel_floored: |dS_ij| / max(|S_ij|, 1e-3 S11).
-    kappa = sqrt(sum_b |A_b|^2) / |sum_b A_b| per bin (FP64 incoherent sum over the
-    coherent one): a per-beam relative error eps becomes ~eps kappa in the bin, so
-    bins of the coherent speckle minima (kappa >> 1) are reported separately and
-    err / kappa gives the per-beam precision of the mixed kernel."""
-    amp64 = np.sqrt((a64.reshape(8, nbins) ** 2).sum(0))
-    kappa = np.sqrt(incoherent) / amp64
-    d = np.abs(mm - m64)
-    s11 = np.abs(m64[0])
-    rel = d / np.maximum(np.abs(m64), 1e-3 * s11)
-    norm = d / s11
+def gate_report(M, M64, th=None, ph=None):
+    """M against the FP64 reference M64 (16 x nbins) under the accuracy gate: per element the
+    floored relative error |dS_ij| / max(|S_ij|, 1e-3 S11) (max, p99.9), S11's relative error,
+    the normalised error |dS_ij| / S11, and the bins that fail."""
+    s11 = np.abs(M64[0])
+    d = np.abs(M - M64)
+    flo = d / np.maximum(np.abs(M64), GATE["floor"] * s11)
     s11r = d[0] / s11
+    fail = (flo[1:] > GATE["elem"]).any(0) | (s11r > GATE["S11"])
     q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
-    out = {"reference": "FP64 GPU kernel on the same prepared beams",
-           "S11_rel": q(s11r), "norm_elem": q(norm), "rel_floored": q(rel),
-           "kappa": {"median": float(np.median(kappa)), "p999": float(np.quantile(kappa, 0.999)),
-                     "max": float(kappa.max())},
-           "per_beam_precision": q(np.abs(np.sqrt(np.abs(mm[0]) / s11) - 1) / kappa),
-           "per_element_norm_max": [float(norm[e].max()) for e in range(16)]}
-    for kmax in (3.0, 10.0):
-        sel = kappa <= kmax
-        out[f"kappa_le_{int(kmax)}"] = {"bins": int(sel.sum()), "S11_rel": q(s11r[sel]),
-                                        "norm_elem": q(norm[:, sel])}
+    out = {"S11_rel": q(s11r), "floored_elem": q(flo[1:]), "norm_elem": q(d[1:] / s11),
+           "floored_per_element_max": [float(flo[e].max()) for e in range(16)],
+           "failing_bins": int(fail.sum()), "bins": int(len(s11)), "meets_gate": bool(not fail.any())}
+    if th is not None and fail.any():
+        idx = np.nonzero(fail)[0][:8]
+        out["first_failing"] = [{"theta_deg": float(np.degrees(th[i // len(ph)])),
+                                 "phi_deg": float(np.degrees(ph[i % len(ph)]))} for i in idx]
     return out
 
 
-def _prepare_trees(w, lam, dev):
+def _records_host(beams, lam, fov_value, dev):
+    """Host preparation (reference operation order, glibc), then resident on the device."""
     import torch
     from abt_amd import lib
-    R = lib.load().abt_record_doubles()
-    nb, ne = w["beams"]["ampl"].shape[0], w["ext"]["ampl"].shape[0]
-    rb = torch.empty((nb, R), dtype=torch.float64, device=dev)
-    re = torch.empty((ne, R), dtype=torch.float64, device=dev)
-    lib.prepare_beams_dev(w["beams"], nb, lam, rb, fov_value=np.pi / 2)
-    lib.prepare_beams_dev(w["ext"], ne, lam, re, fov_value=lib.PI_R4 / 2)
-    return rb, re, nb, ne
+    recs, vrecs = lib.prepare_beams_host(beams, lam, fov_value=fov_value)
+    poly = beams.n > 0 and bool((np.diff(beams.vert_off) > 3).any())   # vertex records only for polygons
+    return torch.from_numpy(recs).to(dev), (torch.from_numpy(vrecs).to(dev) if poly else None)
 
 
-def orientation_average(norient, steps, lam, dev, with_fp64=True, fixed_tiles=None, particle="rough_column",
-                        grid_name="orient", with_kappa=False):
-    """Batched orientation averaging: `norient` orientations (uniform on SO(3), seeded) of
-    a Gaussian-rough hexagonal column (radius 10 um, length 20 um, sigma 0.5 um, 2112
-    facets), 10 internal reflections, 721 theta x 180 phi bins.  The whole job (every
-    orientation's beam and ext trees, per-orientation Mueller matrices summed
-    incoherently, phi integral, per-orientation integrals) is one graph-captured
-    persistent launch with the autotuned tile shape, in mixed precision.  Beside it: the
-    plain FP64 drop-in path run as the reference drivers do (per orientation diff_main,
-    finalise, summation), and the FP64 batched kernel as the accuracy reference.  With
-    particle="large_column", grid_name="large": the absorbing 20,736-facet column (radius
-    50 um, length 100 um, m 1.31+0.01i, 12 reflections) on 451 theta (0.05 deg to 5 deg,
-    then 0.5 deg) x 360 phi bins."""
-    import torch
-    from abt_amd import lib
-    from abt_amd.workloads import GRIDS, grid, orientation_workload
-    t0 = time.time()
-    w = orientation_workload(norient, particle, device=dev)
-    torch.cuda.synchronize()
-    boff, eoff = w["beam_off"].cpu().numpy(), w["ext_off"].cpu().numpy()
-    counts = np.diff(boff) + np.diff(eoff)
-    th, ph = grid(*GRIDS[grid_name])
-    g = lib.DeviceGrid(th, ph, lam)
-    rb, re, nb, ne = _prepare_trees(w, lam, dev)
-    del w
-    torch.cuda.synchronize()
-    nbins = g.nbins
-    bb = (nb + ne) * nbins
-    log(f"orientation average: {norient} orientations, {nb} beams + {ne} ext "
-        f"({counts.min()}-{counts.max()} per orientation), {bb:.4e} beam.bins, "
-        f"generated in {time.time() - t0:.1f} s")
-    f64 = dict(dtype=torch.float64, device=dev)
-    res = {}
+class OrientationCase:
+    """A multi-orientation workload resident on the device: records of both trees, grid,
+    offsets, and the plain FP64 drop-in loop as the accuracy reference."""
 
-    def buffers():
-        return (torch.empty(16 * nbins, **f64), torch.empty(6 * norient, **f64),
-                torch.empty(16 * len(th), **f64))
+    def __init__(self, norient, particle, grid_name, dev):
+        import torch
+        from abt_amd import lib
+        from abt_amd.workloads import GRIDS, grid, orientation_workload, to_beamset
+        t0 = time.time()
+        w = orientation_workload(norient, particle, device=dev)
+        self.boff, self.eoff = w["beam_off"].cpu().numpy(), w["ext_off"].cpu().numpy()
+        B, E = to_beamset(w["beams"]), to_beamset(w["ext"])
+        del w
+        torch.cuda.empty_cache()
+        self.th, self.ph = grid(*GRIDS[grid_name])
+        self.g = lib.DeviceGrid(self.th, self.ph, LAM)
+        self.rb, self.vb = _records_host(B, LAM, np.pi / 2, dev)
+        self.re, self.ve = _records_host(E, LAM, lib.PI_R4 / 2, dev)
+        self.B, self.E = B, E
+        self.nb, self.ne = B.n, E.n
+        self.norient = norient
+        self.nbins = self.g.nbins
+        self.bb = (self.nb + self.ne) * self.nbins
+        self.dev = dev
+        counts = np.diff(self.boff) + np.diff(self.eoff)
+        self.counts = (int(counts.min()), int(counts.max()))
+        log(f"{particle}: {norient} orientations, {self.nb} beams + {self.ne} ext ({counts.min()}-{counts.max()} "
+            f"per orientation), {len(self.th)} x {len(self.ph)} bins, {self.bb:.4e} beam.bins "
+            f"({time.time() - t0:.1f} s to trace and prepare)")
 
-    # mixed precision: autotune (or a fixed tile shape), capture, replay
-    plan = lib.BatchPlan(g, boff, eoff, "mixed")
-    if fixed_tiles:
-        tb, bpt = (int(v) for v in fixed_tiles.lower().split("x"))
-        plan.set_tiles(tb, bpt)
-        tune = {}
-    else:
-        tune = plan.autotune(rb, re, sample=32)
-    tiles = plan.tiles()
-    m, I, m1 = buffers()
-    s = torch.cuda.Stream()
-    with torch.cuda.stream(s):
-        plan.capture(rb, re, m, I, m1, stream=s)
-    plan.launch(stream=s)
-    s.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(steps):
-        plan.launch(stream=s)
-    e1.record(s)
-    s.synchronize()
-    ms = e0.elapsed_time(e1) / steps
-    v_mixed = bb / (ms * 1e-3)
-    log(f"  mixed batched (graph): {ms:.1f} ms/step, {v_mixed:.4e} beam.bin/s, tiles {tiles}")
-    mm, Im = m.cpu().numpy().reshape(16, -1), I.cpu().numpy().reshape(-1, 6)
-    res = {
-        "workload": (f"{norient} orientations (uniform SO(3), seed 1) of a rough hexagonal column "
-                     "(r 10 um, l 20 um, sigma 0.5 um, 2112 facets), m 1.31, 10 reflections, "
-                     "721 theta x 180 phi bins, lambda 0.532 um") if particle == "rough_column" else
-                    (f"{norient} orientations (uniform SO(3), seed 1) of the absorbing hexagonal column "
-                     "(r 50 um, l 100 um, 20736 facets), m 1.31+0.01i, 12 reflections, "
-                     f"{len(th)} theta x {len(ph)} phi bins, lambda 0.532 um"),
-        "beams": int(nb), "ext_beams": int(ne), "beams_per_orientation": [int(counts.min()), int(counts.max())],
-        "beam_bins": bb, "metric": "beam·bin diffractions/s", "value": v_mixed, "ms_per_step": ms,
-        "steps": steps, "launch": "one hipGraph replay per step (persistent kernel + reductions)",
-        "tiles": tiles, "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in tune.items()},
-        "roofline": {"bound": "valu", "achieved": F_ALG * bb / (ms * 1e-3) / 1e12, "peak": 157.3,
-                     "unit": "TFLOP/s", "frac": F_ALG * bb / (ms * 1e-3) / PEAK["mixed"],
-                     "kernel": "k_batch<1,%d,%d>" % (tiles["beams_per_tile"], tiles["bins_per_thread"]),
-                     "traffic": _traffic("k_batch<1,%d,%d>" % (tiles["beams_per_tile"], tiles["bins_per_thread"]))
-                     if norient == 1000 else None},
-    }
-    if not with_fp64:
-        plan.close()
-        return res
-    # FP64 batched (accuracy reference)
-    plan64 = lib.BatchPlan(g, boff, eoff, "fp64")
-    m64, I64, m164 = buffers()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    plan64.run(rb, re, m64, I64, m164)
-    torch.cuda.synchronize()
-    ms64b = (time.perf_counter() - t) * 1e3
-    log(f"  fp64 batched: {ms64b:.1f} ms, {bb / (ms64b * 1e-3):.4e} beam.bin/s")
-    M64, I64h = m64.cpu().numpy().reshape(16, -1), I64.cpu().numpy().reshape(-1, 6)
-    # plain FP64 drop-in path, orientation by orientation
-    ab, ae = torch.empty(8 * nbins, **f64), torch.empty(8 * nbins, **f64)
-    mu, m1d, par = torch.empty(48 * nbins, **f64), torch.empty(48 * len(th), **f64), torch.empty(16, **f64)
-    tot = torch.zeros(16 * nbins, **f64)
-    ws = max(lib.far_field_workspace(g, int(max(np.diff(boff).max(), np.diff(eoff).max())), "fp64"), 8)
-    work = torch.empty(ws // 8 + 1, **f64)
-    fwork = torch.empty(lib.load().abt_finalise_workspace(g.handle) // 8 + 1, **f64)
+    def buffers(self):
+        import torch
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        return (torch.empty(16 * self.nbins, **f64), torch.empty(6 * self.norient, **f64),
+                torch.empty(16 * len(self.th), **f64))
 
-    def plain(o):
-        b0, b1, q0, q1 = int(boff[o]), int(boff[o + 1]), int(eoff[o]), int(eoff[o + 1])
-        lib.far_field_dev(g, rb[b0:b1], b1 - b0, ab, precision="fp64", work=work)
-        lib.far_field_dev(g, re[q0:q1], q1 - q0, ae, precision="fp64", work=work)
-        lib.finalise_dev(g, ab, ae, mu, m1d, par, work=fwork)
-        tot.add_(mu[:16 * nbins])
+    def plan(self, precision, autotune=True):
+        from abt_amd import lib
+        p = lib.BatchPlan(self.g, self.boff, self.eoff, precision)
+        tune = p.autotune(self.rb, self.re, self.vb, self.ve, sample=32) if autotune else {}
+        return p, tune
 
-    plain(0)
-    tot.zero_()
-    torch.cuda.synchronize()
-    t = time.perf_counter()
-    for o in range(norient):
-        plain(o)
-    torch.cuda.synchronize()
-    dt_plain = time.perf_counter() - t
-    v_plain = bb / dt_plain
-    log(f"  fp64 plain loop: {dt_plain * 1e3:.1f} ms, {v_plain:.4e} beam.bin/s")
-    Mp = tot.cpu().numpy().reshape(16, -1)
-    kappa = None
-    if with_kappa:
-        # conditioning of the coherent sums: kappa^2 = sum_o sum_b |A_b|^2 / sum_o |sum_b A_b|^2 (FP64),
-        # with sum_o |X_o|_F^2 = 2 M11 for X = k A (the Mueller matrix's 1/2 convention)
-        incs = torch.zeros(nbins, **f64)
-        tmp = torch.empty(nbins, **f64)
-        for o in range(norient):
-            for rr, oo in ((rb, boff), (re, eoff)):
-                a0, a1 = int(oo[o]), int(oo[o + 1])
-                if a1 > a0:
-                    lib.far_field_incoherent_dev(g, rr[a0:a1], a1 - a0, tmp)
-                    incs.add_(tmp)
+    def timed_graph(self, precision, steps, warmup, with_kernel_ms=True):
+        """Replays of the plan's graph: (ms per step, kernel ms from HIP events on the plan's
+        stream around an eager run, Mueller sum, integrals, plan info)."""
+        import torch
+        p, tune = self.plan(precision)
+        m, I, m1 = self.buffers()
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            p.capture(self.rb, self.re, m, I, m1, self.vb, self.ve, stream=s)
+        for _ in range(max(warmup, 1)):
+            p.launch(stream=s)
+        s.synchronize()
         torch.cuda.synchronize()
-        kappa = np.sqrt(incs.cpu().numpy() * (2 * lib.PI_R4 / lam) ** 2 / (2 * np.abs(M64[0])))
-    s11 = np.abs(M64[0])
-    norm = np.abs(mm - M64) / s11
-    s11r = np.abs(mm[0] - M64[0]) / s11
-    # the acceptance metric |dS_ij| / max(1e-3 |S11|, |S_ij|) (floor 1e-3 S11), per element
-    flo = np.abs(mm - M64) / np.maximum(1e-3 * s11, np.abs(M64))
-    q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
-    bad_e = np.nonzero((flo[1:] > 1e-5).any(0))[0]
-    bad_s = np.nonzero(s11r > 1e-6)[0]
-    asym_m, asym_64 = Im[:, 1] / Im[:, 0], I64h[:, 1] / I64h[:, 0]
-    res.update({
-        "fp64_plain": {"value": v_plain, "ms": dt_plain * 1e3,
-                       "path": "per orientation: abt_far_field_dev x2 + abt_finalise_dev + sum"},
-        "fp64_batched": {"value": bb / (ms64b * 1e-3), "ms": ms64b},
-        "speedup_vs_fp64_plain": v_mixed / v_plain,
-        "accuracy": {"reference": "FP64 batched kernel on the same prepared beams",
-                     "S11_rel": q(s11r), "norm_elem": q(norm),
-                     "per_element_norm_max": [float(norm[e].max()) for e in range(16)],
-                     "rel_floored": {"floor": "1e-3 S11", "per_element_max": [float(flo[e].max()) for e in range(16)],
-                                     "per_element_p999": [float(np.quantile(flo[e], 0.999)) for e in range(16)]},
-                     "flagged_bins": {"elements_gt_1e-5": int(len(bad_e)), "S11_gt_1e-6": int(len(bad_s)),
-                                      "first": [int(x) for x in np.union1d(bad_e, bad_s)[:20]]},
-                     **({"kappa": _kappa_report(kappa, flo, s11r, s11, th, ph)} if kappa is not None else {}),
-                     "plain_vs_batched_fp64_norm_max": float((np.abs(Mp - M64) / s11).max()),
-                     "asymmetry_abs_max": float(np.abs(asym_m - asym_64).max()),
-                     "scatt_rel_max": float(np.abs(Im[:, 0] / I64h[:, 0] - 1).max())},
-    })
-    for x in (plan, plan64):
-        x.close()
-    return res
+        t = time.perf_counter()
+        for _ in range(steps):
+            p.launch(stream=s)
+        s.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / steps
+        kms = None
+        if with_kernel_ms:
+            # the persistent kernel and its short reductions, eager on the same stream
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(s)
+            p.run(self.rb, self.re, m, I, m1, self.vb, self.ve, stream=s)
+            e1.record(s)
+            s.synchronize()
+            kms = e0.elapsed_time(e1)
+        info = {"tiles": p.tiles(), "autotune_ms": {f"{k[0]}x{k[1]}": v for k, v in tune.items() if v}}
+        if precision == "mixed_gated":
+            info["refined_bins"] = p.refined_bins(stream=s)
+        M = m.cpu().numpy().reshape(16, -1)
+        Ih = I.cpu().numpy().reshape(-1, 6)
+        p.close()
+        return ms, kms, M, Ih, info
+
+    def plain_fp64(self):
+        """The plain FP64 drop-in path orientation by orientation, as the reference drivers
+        run it: per orientation abt_far_field_dev for both trees, abt_finalise_dev, sum."""
+        import torch
+        from abt_amd import lib
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        nbins = self.nbins
+        ab, ae = torch.empty(8 * nbins, **f64), torch.empty(8 * nbins, **f64)
+        mu, m1d, par = torch.empty(48 * nbins, **f64), torch.empty(48 * len(self.th), **f64), torch.empty(16, **f64)
+        ws = max(lib.far_field_workspace(self.g, int(max(np.diff(self.boff).max(), np.diff(self.eoff).max())),
+                                         "fp64"), 8)
+        work = torch.empty(ws // 8 + 1, **f64)
+        fwork = torch.empty(lib.load().abt_finalise_workspace(self.g.handle) // 8 + 1, **f64)
+        tot = torch.zeros(16 * nbins, **f64)
+        params = []
+
+        def one(o):
+            b0, b1, q0, q1 = int(self.boff[o]), int(self.boff[o + 1]), int(self.eoff[o]), int(self.eoff[o + 1])
+            lib.far_field_dev(self.g, self.rb[b0:b1], b1 - b0, ab, vrecs=self.vb, precision="fp64", work=work)
+            lib.far_field_dev(self.g, self.re[q0:q1], q1 - q0, ae, vrecs=self.ve, precision="fp64", work=work)
+            lib.finalise_dev(self.g, ab, ae, mu, m1d, par, work=fwork)
+            tot.add_(mu[:16 * nbins])
+
+        one(0)
+        tot.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for o in range(self.norient):
+            one(o)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        return dt * 1e3, tot.cpu().numpy().reshape(16, -1)
 
 
-def _kappa_report(kappa, flo, s11r, s11, th, ph):
-    """Errors against the conditioning kappa of each bin's coherent sums (a per-beam relative
-    error eps shows up as ~eps kappa): quantiles of kappa, the share of flagged bins that are
-    ill-conditioned (kappa > 10), the errors divided by max(kappa, 1) (the per-beam precision;
-    kappa < 1 where the beams add coherently), the flagged bins by the depth of S11 below its
-    largest value, and where the largest errors sit."""
-    q = lambda x: {"max": float(x.max()), "p999": float(np.quantile(x, 0.999))}
-    smax = s11.max()
-    where = lambda i: {"theta_deg": float(np.degrees(th[i // len(ph)])), "phi_deg": float(np.degrees(ph[i % len(ph)])),
-                       "kappa": float(kappa[i]), "S11_rel": float(s11r[i]), "S11_over_max": float(s11[i] / smax)}
-    worst = flo[1:].max(0)
-    flag = (worst > 1e-5) | (s11r > 1e-6)
-    well = kappa <= 10
-    return {"median": float(np.median(kappa)), "p99": float(np.quantile(kappa, 0.99)), "max": float(kappa.max()),
-            "flagged_with_kappa_gt_10": int((flag & ~well).sum()), "flagged_with_kappa_le_10": int((flag & well).sum()),
-            "well_conditioned_bins": int(well.sum()),
-            "S11_rel_kappa_le_10": q(s11r[well]) if well.any() else None,
-            "floored_kappa_le_10": q(worst[well]) if well.any() else None,
-            "S11_rel_over_kappa": q(s11r / np.maximum(kappa, 1.0)),
-            "floored_over_kappa": q(worst / np.maximum(kappa, 1.0)),
-            "flagged_by_S11_level": {"S11_ge_1e-4_max": int((flag & (s11 >= 1e-4 * smax)).sum()),
-                                     "S11_1e-6_to_1e-4_max": int((flag & (s11 < 1e-4 * smax) & (s11 >= 1e-6 * smax)).sum()),
-                                     "S11_lt_1e-6_max": int((flag & (s11 < 1e-6 * smax)).sum())},
-            "bins_by_S11_level": {"S11_ge_1e-4_max": int((s11 >= 1e-4 * smax).sum()),
-                                  "S11_1e-6_to_1e-4_max": int(((s11 < 1e-4 * smax) & (s11 >= 1e-6 * smax)).sum()),
-                                  "S11_lt_1e-6_max": int((s11 < 1e-6 * smax).sum())},
-            "worst_S11_bins": [where(i) for i in np.argsort(-s11r)[:5]],
-            "worst_S11_over_kappa_bins": [where(i) for i in np.argsort(-(s11r / np.maximum(kappa, 1.0)))[:5]]}
+def orientation_section(case, precision, steps, warmup, M64, plain_ms, gate=True):
+    ms, kms, M, Ih, info = case.timed_graph(precision, steps, warmup)
+    value = case.bb / (ms * 1e-3)
+    peak = PEAK[PEAK_OF[precision]]
+    kname = {"fp64_fast": "k_batch<4,%d,1>", "mixed": "k_batch<1,%d,%d>",
+             "mixed_gated": "k_batch<1,%d,%d> (estimate) + k_batch<4,16,1> (listed bins)"}[precision]
+    t = info["tiles"]
+    kname = kname % ((t["beams_per_tile"],) if precision == "fp64_fast" else (t["beams_per_tile"], t["bins_per_thread"]))
+    out = {"value": value, "unit": "beam·bin/s", "ms_per_step": ms, "steps": steps,
+           "speedup_vs_fp64_plain": plain_ms / ms, **info,
+           "roofline": {"bound": "valu", "achieved": F_ALG * case.bb / (kms * 1e-3) / 1e12, "peak": peak / 1e12,
+                        "unit": "TFLOP/s", "frac": F_ALG * case.bb / (kms * 1e-3) / peak,
+                        "traffic": _traffic(kname), "kernel": kname, "kernel_ms": kms,
+                        "flop_per_beam_bin": F_ALG, "beam_bins_per_launch": case.bb}}
+    if M64 is not None:
+        out["accuracy"] = gate_report(M, M64, case.th, case.ph)
+    log(f"  {precision}: {ms:.1f} ms/step, {value:.4e} beam.bin/s, {plain_ms / ms:.2f}x the FP64 plain loop"
+        + (f", gate {'met' if out['accuracy']['meets_gate'] else 'MISSED'} "
+           f"(floored max {out['accuracy']['floored_elem']['max']:.2e}, S11 {out['accuracy']['S11_rel']['max']:.2e})"
+           if M64 is not None else "") + (f", refined {info['refined_bins']} bins" if "refined_bins" in info else ""))
+    return out, M, Ih
+
+
+def cpu_baseline_orient(case, nsample_orient):
+    """The C restatement of diff_main + finalise (oracle/, FP64, OpenMP over theta columns) on
+    the first orientations of the workload, on the full grid."""
+    import oracle
+    cores = min(16, os.cpu_count() or 1)
+    bb = 0
+    t = time.perf_counter()
+    for o in range(nsample_orient):
+        b = case.B.subset(np.arange(case.boff[o], case.boff[o + 1]))
+        e = case.E.subset(np.arange(case.eoff[o], case.eoff[o + 1]))
+        rb, re = oracle.diff_main(case.th, case.ph, LAM, b, e, nthreads=cores, raw=True)
+        oracle.finalise(case.th, case.ph, LAM, rb, re)
+        bb += (b.n + e.n) * case.nbins
+    dt = time.perf_counter() - t
+    return {"value": bb / dt, "unit": "beam·bin/s", "cores": cores, "kind": "port",
+            "sample": f"the first {nsample_orient} orientations of the same workload on the full grid "
+                      f"({bb:.3e} beam.bins, {dt:.1f} s): diff_main + finalise of the C restatement of the "
+                      "reference (oracle/diff_oracle.c; the reference itself needs a stand-in for GNU rand() to "
+                      "build here, so it is not run)"}
+
+
+def stress_section(nbeam, steps, lam, dev, cpu_sample):
+    """The 200,000-aperture stress test: one orientation, 1801 x 360 bins.  Per precision
+    one step = abt_far_field_dev over every aperture + abt_finalise_dev (records prepared
+    on the host, resident in HBM); accuracy of the Mueller matrix against the FP64 kernel."""
+    import torch
+    from abt_amd import lib
+    from abt_amd.beams import BeamSet
+    from abt_amd.workloads import GRIDS, grid, stress_beams
+    th, ph = grid(*GRIDS["stress"])
+    t0 = time.time()
+    beams = stress_beams(nbeam)
+    g = lib.DeviceGrid(th, ph, lam)
+    recs, vrecs = _records_host(beams, lam, None, dev)
+    nbins = g.nbins
+    log(f"stress: {nbeam} apertures, {len(th)} x {len(ph)} bins ({time.time() - t0:.1f} s to generate and prepare)")
+    f64 = dict(dtype=torch.float64, device=dev)
+    amp, amp_ext = torch.empty(8 * nbins, **f64), torch.zeros(8 * nbins, **f64)
+    mu, m1d, par = torch.empty(48 * nbins, **f64), torch.empty(48 * len(th), **f64), torch.empty(16, **f64)
+    ws = lib.far_field_workspace(g, nbeam, "fp64")
+    work = torch.empty(max(ws, 8) // 8 + 1, **f64)
+    fwork = torch.empty(lib.load().abt_finalise_workspace(g.handle) // 8 + 1, **f64)
+    st = torch.cuda.current_stream()
+    out = {"workload": f"diffraction stress test: {nbeam} seeded triangular apertures (edge directions uniform "
+                       f"in the aperture plane), {len(th)} theta x {len(ph)} phi bins, lambda {lam} um, one orientation",
+           "beams": nbeam, "bins": nbins, "beam_bins": nbeam * nbins}
+    mus = {}
+    for prec in ("mixed", "fp64_fast", "fp64"):
+        kms = []
+        for rep in range(steps + 1):
+            e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            e0.record(st)
+            lib.far_field_dev(g, recs, nbeam, amp, vrecs=vrecs, precision=prec, work=work, stream=st)
+            e1.record(st)
+            amp_ext.zero_()
+            lib.finalise_dev(g, amp, amp_ext, mu, m1d, par, work=fwork, stream=st)
+            e2.record(st)
+            torch.cuda.synchronize()
+            if rep:   # the first launch warms up
+                kms.append((e0.elapsed_time(e1), e0.elapsed_time(e2)))
+        k = float(np.mean([a for a, _ in kms]))
+        s = float(np.mean([b for _, b in kms]))
+        peak = PEAK[PEAK_OF[prec]]
+        kname = {"mixed": "k_far_field<1,16>", "fp64_fast": "k_far_field<4,16>", "fp64": "k_far_field<0,16>"}[prec]
+        out[prec] = {"value": nbeam * nbins / (s * 1e-3), "ms_per_step": s, "kernel_ms": k,
+                     "roofline": {"bound": "valu", "achieved": F_ALG * nbeam * nbins / (k * 1e-3) / 1e12,
+                                  "peak": peak / 1e12, "unit": "TFLOP/s",
+                                  "frac": F_ALG * nbeam * nbins / (k * 1e-3) / peak, "kernel": kname,
+                                  "traffic": _traffic(kname) if nbeam == 200_000 else None}}
+        mus[prec] = mu[:16 * nbins].cpu().numpy().reshape(16, -1)
+        log(f"  stress {prec}: {s:.1f} ms/step (kernel {k:.1f} ms), {nbeam * nbins / (s * 1e-3):.4e} beam.bin/s")
+    for prec in ("mixed", "fp64_fast"):
+        out[prec]["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out[prec]["ms_per_step"]
+        out[prec]["accuracy"] = gate_report(mus[prec], mus["fp64"])
+    if cpu_sample > 0:
+        import oracle
+        cores = min(16, os.cpu_count() or 1)
+        sub = beams.subset(np.arange(min(cpu_sample, nbeam)))
+        t = time.perf_counter()
+        oracle.diff_main(th, ph, lam, sub, BeamSet.empty(), nthreads=cores, raw=True)
+        dt = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": sub.n * nbins / dt, "unit": "beam·bin/s", "cores": cores, "kind": "port",
+                               "sample": f"first {sub.n} apertures on the full grid ({sub.n * nbins:.3e} beam.bins, "
+                                         f"{dt:.1f} s): diff_main of the C restatement of the reference, OpenMP"}
+    return out
+
+
+def dropin_section(lam, dev):
+    """The reference-facing host entry points on the reference's CPU-runnable case: the
+    default hexagonal column (r 5 um, l 10 um, -tri) at Euler (0, 30, 0), 8 reflections,
+    361 x 180 bins.  abt_diff_main per call (host beams in, host amplitudes out: the
+    "diffraction took" span of diff_mod.f90:818-822) and abt_diffraction per beam (the
+    -memory mode call, beam_loop_mod.f90:37-51), after the first call has cached the grid."""
+    from abt_amd import lib
+    from abt_amd.workloads import GRIDS, column_beams, euler_matrix, grid, hex_column
+    v, f = hex_column(5.0, 10.0, 6, 12, 0.0)
+    b, e, _ = column_beams(v, f, euler_matrix(0, 30, 0), rec=8)
+    th, ph = grid(*GRIDS["small"])
+    x, y, z = lib.make_far_field_bins(th, ph)
+    nbins = len(th) * len(ph)
+    out = {"workload": f"default hexagonal column (r 5 um, l 10 um) at Euler (0, 30, 0), 8 reflections: "
+                       f"{b.n} beams + {e.n} ext beams, {len(th)} x {len(ph)} bins", "beams": b.n, "ext_beams": e.n}
+    for prec in ("fp64", "fp64_fast", "mixed"):
+        lib.diff_main(th, ph, lam, b, e, precision=prec, raw=True)
+        t = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            lib.diff_main(th, ph, lam, b, e, precision=prec, raw=True)
+        dm = (time.perf_counter() - t) / reps
+        lib.diffraction(th, ph, x, y, z, lam, b, 0, precision=prec)
+        t = time.perf_counter()
+        nd = 50
+        for i in range(nd):
+            lib.diffraction(th, ph, x, y, z, lam, b, i % b.n, precision=prec)
+        dd = (time.perf_counter() - t) / nd
+        out[prec] = {"diff_main_ms": dm * 1e3, "diff_main_beam_bins_per_s": (b.n + e.n) * nbins / dm,
+                     "diffraction_ms_per_beam": dd * 1e3}
+        log(f"  drop-in {prec}: diff_main {dm * 1e3:.1f} ms, diffraction {dd * 1e3:.2f} ms per beam")
+    return out
+
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed steps of the headline plan")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--orient", type=int, default=1000, help="orientations of the headline workload")
+    ap.add_argument("--precision", default="fp64_fast", choices=["fp64_fast", "mixed_gated", "mixed"],
+                    help="headline plan")
+    ap.add_argument("--section-steps", type=int, default=2, help="timed steps of the other sections")
+    ap.add_argument("--others", default="mixed,mixed_gated", help="other plans timed on the headline workload")
+    ap.add_argument("--plain", type=int, default=1, help="run the plain FP64 loop (speedup and accuracy reference)")
+    ap.add_argument("--beams", type=int, default=200_000, help="apertures of the stress test (0: skip)")
+    ap.add_argument("--large-orient", type=int, default=24,
+                    help="orientations of the large absorbing column (20,736 facets; 0: skip)")
+    ap.add_argument("--adjoint-orient", type=int, default=64, help="orientations of the adjoint workload (0: skip)")
+    ap.add_argument("--dropin", type=int, default=1, help="time the reference-facing host entry points (0: skip)")
+    ap.add_argument("--vis-beams", type=int, default=4096, help="beams of the facet-visibility workload (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=2,
+                    help="orientations of the CPU baseline sample (0: skip; the stress test samples 96 apertures)")
+    ap.add_argument("--json-out", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): every GPU box of this build has "
+              "one GPU; the multi-GPU path (orientations sharded by beam count, one fixed-order reduction of "
+              "the Mueller partials) is a design, not built.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    dev = torch.device("cuda:0")
+    out = {"metric": METRIC, "unit": "beam·bin/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    sections = {}
+    if args.orient > 0:
+        case = OrientationCase(args.orient, "rough_column", "orient", dev)
+        plain_ms, M64 = case.plain_fp64() if args.plain else (None, None)
+        if plain_ms:
+            log(f"  fp64 plain loop: {plain_ms:.1f} ms, {case.bb / (plain_ms * 1e-3):.4e} beam.bin/s")
+        head, Mh, Ih = orientation_section(case, args.precision, args.steps, args.warmup, M64, plain_ms or np.nan)
+        out.update({"value": head.pop("value"), "ms_per_step": head.pop("ms_per_step"),
+                    "dtype": {"fp64_fast": "f64", "mixed": "f32+f64", "mixed_gated": "f32+f64"}[args.precision]})
+        head.pop("unit"); head.pop("steps")
+        out["config"] = {"workload": f"batched orientation average: {args.orient} orientations (uniform SO(3), "
+                                     "seed 1) of a Gaussian-rough hexagonal column (r 10 um, l 20 um, sigma 0.5 um, "
+                                     "2112 facets), m 1.31, 10 reflections, 721 theta x 180 phi bins, lambda 0.532 um",
+                         "beams": case.nb, "ext_beams": case.ne, "beams_per_orientation": list(case.counts),
+                         "bins": case.nbins, "beam_bins": case.bb, "precision": args.precision,
+                         "step": "one HIP-graph replay of the batched plan (persistent kernel + reductions)",
+                         "records": "prepared on the host (reference operation order and libm), resident in HBM"}
+        out["roofline"] = head.pop("roofline")
+        out.update(head)
+        if plain_ms:
+            out["fp64_plain"] = {"value": case.bb / (plain_ms * 1e-3), "ms": plain_ms,
+                                 "path": "per orientation: abt_far_field_dev x2 (ABT_FP64) + abt_finalise_dev + sum"}
+        for prec in [p for p in args.others.split(",") if p and p != args.precision]:
+            sec, _, _ = orientation_section(case, prec, args.section_steps, 1, M64, plain_ms or np.nan)
+            sections[prec] = sec
+        if args.cpu_sample > 0:
+            out["cpu_baseline"] = cpu_baseline_orient(case, args.cpu_sample)
+            log(f"  cpu baseline: {out['cpu_baseline']['value']:.4e} beam.bin/s on {out['cpu_baseline']['cores']} cores")
+        del case
+        torch.cuda.empty_cache()
+    if args.beams > 0:
+        sections["stress"] = stress_section(args.beams, args.section_steps, LAM, dev, 96 if args.cpu_sample else 0)
+        torch.cuda.empty_cache()
+    if args.large_orient > 0:
+        case = OrientationCase(args.large_orient, "large_column", "large", dev)
+        plain_ms, M64 = case.plain_fp64()
+        log(f"  large column fp64 plain loop: {plain_ms:.1f} ms")
+        lc = {"workload": f"{args.large_orient} orientations (uniform SO(3), seed 1) of the absorbing hexagonal "
+                          "column (r 50 um, l 100 um, 20736 facets), m 1.31+0.01i, 12 reflections, "
+                          f"{len(case.th)} theta (0.05 deg to 5 deg, then 0.5 deg) x {len(case.ph)} phi bins",
+              "beams": case.nb, "ext_beams": case.ne, "beam_bins": case.bb,
+              "fp64_plain": {"value": case.bb / (plain_ms * 1e-3), "ms": plain_ms}}
+        Ms = {}
+        for prec in ("fp64_fast", "mixed"):
+            lc[prec], Ms[prec], _ = orientation_section(case, prec, args.section_steps, 1, M64, plain_ms)
+        # the frame-free FP64 model (no kxx/kyy clamps, no split-sum cancellation) as a second
+        # reference: on this particle S11 spans 11 decades and the plain kernel's formula is
+        # not exact in the deepest minima
+        p, _ = case.plan("fp64_accurate", autotune=False)
+        m, I, m1 = case.buffers()
+        t = time.perf_counter()
+        p.run(case.rb, case.re, m, I, m1, case.vb, case.ve)
+        torch.cuda.synchronize()
+        Mm = m.cpu().numpy().reshape(16, -1)
+        lc["vs_fp64_model"] = {"model": "batched plan precision fp64_accurate (adjbody.h), "
+                                        f"{(time.perf_counter() - t) * 1e3:.0f} ms",
+                               "fp64_plain": gate_report(M64, Mm, case.th, case.ph),
+                               "fp64_fast": gate_report(Ms["fp64_fast"], Mm, case.th, case.ph),
+                               "mixed": gate_report(Ms["mixed"], Mm, case.th, case.ph)}
+        log("  large column against the FP64 model: " + ", ".join(
+            f"{k} floored max {v['floored_elem']['max']:.2e} ({v['failing_bins']} failing bins)"
+            for k, v in lc["vs_fp64_model"].items() if k != "model"))
+        p.close()
+        sections["large_column"] = lc
+        del case
+        torch.cuda.empty_cache()
+    if args.adjoint_orient > 0:
+        sections["adjoint"] = adjoint_bench(args.adjoint_orient, LAM, dev)
+    if args.dropin:
+        sections["dropin"] = dropin_section(LAM, dev)
+    if args.vis_beams > 0:
+        sections["visibility"] = visibility_bench(args.vis_beams, 20, 2, dev, args.cpu_sample > 0)
+    out.update(sections)
+    line = json.dumps(out)
+    if args.json_out:
+        with open(args.json_out, "w") as f:
+            f.write(line + "\n")
+    print(line, flush=True)
 
 
 def adjoint_bench(norient, lam, dev):
@@ -583,20 +607,6 @@ def vis_cpu_baseline(wl, nsample, gpu_value):
             "sample": f"the first {n} beams of the same workload, find_vis_int restated in C, 1 thread",
             "gpu_over_cpu": gpu_value / val}
 
-
-def cpu_baseline(beams, th, ph, lam, nsample):
-    """The FP64 C oracle (OpenMP over theta columns) on the first `nsample` apertures."""
-    import oracle
-    from abt_amd.beams import BeamSet
-    cores = min(16, os.cpu_count() or 1)
-    sub = beams.subset(np.arange(min(nsample, beams.n)))
-    t = time.perf_counter()
-    oracle.diff_main(th, ph, lam, sub, BeamSet.empty(), nthreads=cores, raw=True)
-    dt = time.perf_counter() - t
-    bb = sub.n * len(th) * len(ph)
-    return {"value": bb / dt, "unit": "beam·bin/s", "cores": cores, "kind": "port",
-            "sample": f"first {sub.n} apertures of the stress set on the full 1801x360 grid "
-                      f"({bb:.3e} beam.bins, {dt:.1f} s, FP64 C restatement of diff_mod, OpenMP)"}
 
 
 if __name__ == "__main__":

@@ -240,6 +240,16 @@ int abt_batch_launch(const abt_batch_plan* plan, void* stream);
 /* ABT_MIXED_GATED plans: the number of bins the last run recomputed in FP64 (0 for other
  * precisions).  Synchronises the stream. */
 int abt_batch_gate_stats(const abt_batch_plan* plan, int32_t* refined_bins, void* stream);
+/* -scaling of finalise (outputs_mod.f90:146-184) in the batched plan: in every orientation o
+ * the beam amplitudes are divided by sqrt(scatt_beam / beam_energy_out[o]) and the ext
+ * amplitudes by sqrt(scatt_ext_diff / ext_energy_out[o]), scatt_* being that orientation's
+ * own cross sections, before the Mueller sum; the integrals are those of the divided
+ * amplitudes.  beam_energy_out, ext_energy_out: device arrays of norient doubles, read by
+ * every run (a captured graph reads their values at replay); NULL, NULL turns scaling off.
+ * The plan then keeps every orientation's amplitudes: 128 bytes per bin per orientation,
+ * allocated here (not in abt_batch_run).  Drops a captured graph.  Not for ABT_MIXED_GATED. */
+int abt_batch_set_scaling(abt_batch_plan* plan, const double* beam_energy_out,
+                          const double* ext_energy_out);
 
 /* ---- adjoint: gradients for shape and roughness retrieval ---------------------
  * No reference counterpart (the reference has no derivatives); the forward it

@@ -49,7 +49,8 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
                     }
             }
         }
-        if (prec == 1 || prec == 4) mixed_finish(acc, cphi[iphi], sphi[iphi]);
+        if (prec == 1) mixed_finish(acc, cphi[iphi], sphi[iphi]);
+        if (prec == 4) fp64r_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
         if (prec == 3) model_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
         for (int q = 0; q < 4; q++) {
             out[2 * (bin + (int64_t)nbins * q)] = acc[2 * q];

@@ -76,3 +76,63 @@ def test_emu_mixed_reflections_near_forward(orc):
     ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=8)
     err = ampl_rel_err(_run(1, th, ph, bs), ref)
     assert err.max() <= 1e-6, summary(err)
+
+
+def _fp64r_check(got, ref, fwd, model, tol=1e-7):
+    """The restructured FP64 body against the oracle: per bin to tol of the bin (floored at
+    1e-3 of the largest), the oracle's own split-formula rounding setting the level.  Along a
+    beam's exact forward direction (|q| a < 1e-3) the oracle loses ~1e-16 / (|q| a)^2; there
+    the body is checked against the frame-free FP64 model (adjbody.h; emulator precision 3),
+    an independent FP64 evaluation without that cancellation, to 1e-8."""
+    err = ampl_rel_err(got, ref)
+    assert err[~fwd].max() <= tol, summary(err[~fwd])
+    if fwd.any():
+        err_m = ampl_rel_err(got, model)
+        assert err_m[fwd].max() <= 1e-8, summary(err_m[fwd])
+
+
+def test_emu_fp64r_body_matches_oracle(orc):
+    th, ph = _coarse()
+    b, e = _column_case()
+    rb, re = orc.diff_main(th, ph, LAM, b, e, nthreads=8)
+    _fp64r_check(_run(4, th, ph, b), rb, _forward_mask(th, ph, [b]), _run(3, th, ph, b))
+    _fp64r_check(_run(4, th, ph, e, O.PI_R4 / 2), re, _forward_mask(th, ph, [e]), _run(3, th, ph, e, O.PI_R4 / 2))
+    th, ph = grid((0, 2, 180), (0, 6, 354))
+    for bs in (stress_beams(150, seed=7), random_triangles(200, seed=4, box=40.0)):
+        ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=8)
+        _fp64r_check(_run(4, th, ph, bs), ref, _forward_mask(th, ph, [bs]), _run(3, th, ph, bs))
+
+
+def test_emu_fp64r_edges_normal_to_q(orc):
+    """Bins where an edge is (nearly) normal to q, so its two vertex phases agree to < 1e-3
+    revolution: the difference form of the edge gives way to the product form there.  The
+    triangles have an edge along y in the aperture plane and the bins sweep through the
+    direction whose q is along x."""
+    from tests.formfactor import ff_quadrature  # noqa: F401
+    th = np.radians(np.linspace(20.0, 40.0, 81))
+    ph = np.radians([0.0, 0.05, 90.0, 180.0, 270.0])
+    bs = random_triangles(60, seed=12, box=10.0)
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=8)
+    _fp64r_check(_run(4, th, ph, bs), ref, _forward_mask(th, ph, [bs]), _run(3, th, ph, bs))
+
+
+def test_emu_fp64r_polygons(orc):
+    th, ph = _coarse()
+    rng = np.random.default_rng(8)
+    sets = []
+    for i in range(5):
+        c = rng.uniform(-5, 5, 3)
+        n = rng.normal(size=3); n /= np.linalg.norm(n)
+        a = np.cross(n, rng.normal(size=3)); a /= np.linalg.norm(a)
+        bb = np.cross(n, a)
+        ang = np.sort(rng.uniform(0, 2 * np.pi, 5))
+        rad = rng.uniform(1, 3, 5)
+        verts = c + (rad * np.cos(ang))[:, None] * a + (rad * np.sin(ang))[:, None] * bb
+        prop = n + 0.4 * a; prop /= np.linalg.norm(prop)
+        vk7 = np.cross(prop, a); vk7 /= np.linalg.norm(vk7)
+        sets.append(BeamSet(rng.normal(size=(1, 8)), vk7, prop, [1.0], [0, 5], verts))
+    from abt_amd.beams import concat
+    bs = concat(sets)
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=8)
+    err = ampl_rel_err(_run(4, th, ph, bs), ref)   # the frame-free model takes triangles only
+    assert err.max() <= 1e-7, summary(err)

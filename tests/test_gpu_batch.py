@@ -54,12 +54,17 @@ def _column_orientations(n=4):
             [E.subset(np.arange(eo[o], eo[o + 1])) for o in range(n)])
 
 
-def _oracle_loop(th, ph, bl, el, fast_diff=False):
-    """mueller_total, mueller_1d_total (summation) and per-orientation parameters."""
+def _oracle_loop(th, ph, bl, el, fast_diff=False, energies=None):
+    """mueller_total, mueller_1d_total (summation) and per-orientation parameters;
+    energies = (beam_energy_out[o], ext_energy_out[o]) runs finalise with -scaling."""
     msum, m1sum, params = 0.0, 0.0, []
-    for b, e in zip(bl, el):
+    for o, (b, e) in enumerate(zip(bl, el)):
         rb, re = O.diff_main(th, ph, LAM, b, e, fast_diff=fast_diff, nthreads=8, raw=True)
-        f = O.finalise(th, ph, LAM, rb, re)
+        if energies is None:
+            f = O.finalise(th, ph, LAM, rb, re)
+        else:
+            f = O.finalise(th, ph, LAM, rb, re, scaling=True, beam_energy_out=energies[0][o],
+                           ext_energy_out=energies[1][o])
         msum = msum + f["mueller"]
         m1sum = m1sum + f["mueller_1d"]
         params.append([f["scatt"], f["asymmetry"], f["scatt_beam"], f["asymmetry_beam"],
@@ -223,6 +228,74 @@ def test_fp64_batch_fast_diff_matches_oracle_loop():
     ok = ~fwd & (s11 > 1e-12 * s11.max())
     assert err[ok].max() <= 1e-9, err[ok].max()
     _check_params(_params(I), ref_p, 1e-9)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fast"])
+def test_batch_scaling_matches_oracle_loop(precision):
+    """-scaling (outputs_mod.f90:146-184) in the batched plan against the oracle's loop of
+    finalise(scaling) per orientation, with energies 0.7-1.3 times each orientation's own
+    cross sections: FP64 |dS_ij| / S11 <= 1e-9 off the exact forward directions, the
+    restructured FP64 body <= 1e-6 where the oracle is exact (off those directions, S11 >=
+    1e-4 of the brightest); integrals to 1e-9 / 1e-6; the scaled cross sections equal the
+    energies; a captured graph reads the energies at replay; scaling off restores the
+    unscaled result bit for bit."""
+    import torch
+    th, ph = _coarse()
+    bl, el = _column_orientations()
+    _, _, p0 = _oracle_loop(th, ph, bl, el)
+    rng = np.random.default_rng(17)
+    eb = p0[:, 2] * rng.uniform(0.7, 1.3, len(bl))
+    ee = p0[:, 4] * rng.uniform(0.7, 1.3, len(bl))
+    ref_m, ref_1d, ref_p = _oracle_loop(th, ph, bl, el, energies=(eb, ee))
+    bt = Batch(th, ph, bl, el, precision)
+    m0, I0, _ = bt.run()
+    teb = torch.tensor(eb, dtype=torch.float64, device=bt.dev)
+    tee = torch.tensor(ee, dtype=torch.float64, device=bt.dev)
+    bt.plan.set_scaling(teb, tee)
+    m, I, m1 = bt.run()
+    s11 = np.abs(ref_m[0])
+    fwd = _forward_bins(th, ph, bl + el)
+    err = np.abs(m - ref_m).max(0) / s11
+    if precision == "fp64":
+        assert err[~fwd].max() <= 1e-9, err[~fwd].max()
+        _check_params(_params(I), ref_p, 1e-9)
+        assert np.abs(m1 - ref_1d).max() <= 1e-9 * np.abs(ref_1d[0]).max()
+    else:
+        ok = ~fwd & (s11 >= 1e-4 * s11.max())
+        assert err[ok].max() <= 1e-6, err[ok].max()
+        _check_params(_params(I), ref_p, 1e-6)
+    np.testing.assert_allclose(I[:, 2], eb, rtol=1e-12)
+    np.testing.assert_allclose(I[:, 4], ee, rtol=1e-12)
+    assert np.abs(m - m0).max() > 1e-3 * s11.max()
+    # a captured graph reads the energies when it is replayed
+    s = torch.cuda.Stream()
+    with torch.cuda.stream(s):
+        bt.plan.capture(bt.rb, bt.re, bt.m, bt.i, bt.m1, bt.vb, bt.ve, stream=s)
+    torch.cuda.synchronize()
+    teb.mul_(1.25)
+    torch.cuda.synchronize()
+    for rep in range(2):
+        bt.m.fill_(np.nan)
+        torch.cuda.synchronize()
+        bt.plan.launch(stream=s)
+        s.synchronize()
+        mg = bt.m.cpu().numpy().reshape(16, -1).copy()
+        me, Ie, _ = bt.run()
+        assert np.array_equal(mg, me), rep
+        np.testing.assert_allclose(Ie[:, 2], 1.25 * eb, rtol=1e-12)
+    bt.plan.set_scaling(None, None)
+    m2, I2, _ = bt.run()
+    assert np.array_equal(m2, m0) and np.array_equal(I2, I0)
+
+
+def test_batch_scaling_rejected_for_gated_plan():
+    import torch
+    th, ph = _coarse()
+    g = lib.DeviceGrid(th, ph, LAM)
+    p = lib.BatchPlan(g, [0, 1], [0, 1], "mixed_gated")
+    e = torch.ones(1, dtype=torch.float64, device="cuda:0")
+    with pytest.raises(lib.AbtError, match="gated"):
+        p.set_scaling(e, e)
 
 
 def test_large_absorbing_column():

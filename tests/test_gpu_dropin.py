@@ -65,3 +65,24 @@ def test_grid_cache_follows_inputs(orc):
         ref = orc.diffraction(th, phk, xk, yk, zk, LAM, b, 0)
         got = lib.diffraction(th, phk, xk, yk, zk, LAM, b, 0, precision="fp64")
         _check_fp64(got, ref, _forward_mask(th, phk, [b.subset([0])]))
+
+
+def test_diff_main_on_beams_loaded_from_pbt_export(orc):
+    """The outgoing beams read from a file in the layout of PBT's -export_beam
+    (tests/golden/beam_export.json) through diff_main, with and without -fast_diff (which
+    reads the per-beam field of view the reader forms), against the oracle."""
+    import os
+    from abt_amd.beams import BeamSet, outbeams_from_pbt_json
+    from tests.golden.make_beam_json import case
+    path = os.path.join(os.path.dirname(__file__), "golden", "beam_export.json")
+    V, F, _, _ = case()
+    b = outbeams_from_pbt_json(path, V, F, LAM)
+    th, ph = grid((0, 2, 180), (0, 6, 354))
+    refs = []
+    for fast in (False, True):
+        rb, _ = orc.diff_main(th, ph, LAM, b, BeamSet.empty(), fast_diff=fast, nthreads=8)
+        gb, ge = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision="fp64", fast_diff=fast)
+        _check_fp64(gb, rb, _forward_mask(th, ph, [b]))
+        assert np.all(ge == 0)
+        refs.append(rb)
+    assert np.abs(refs[0] - refs[1]).max() > 1e-3 * np.abs(refs[0]).max()   # the fields of view cull

@@ -129,3 +129,33 @@ def test_beam_set_files_round_trip(tmp_path):
     empty = tmp_path / "e.json"
     BeamSet.empty().to_json(empty)
     assert BeamSet.from_json(empty).n == 0
+
+
+def test_outbeams_from_pbt_beam_export():
+    """The reference's -export_beam file (tests/golden/beam_export.json, written in its layout
+    by tests/golden/make_beam_json.py from a seeded trace): one outgoing beam per illuminating
+    facet of each external beam, internal beams skipped, amplitudes / e-perp / prop / facet
+    vertices exactly as traced, and fov = 8 lambda / d with d the diameter of the circle of
+    the beam's summed facet area (add_to_outbeam_tree, beam_loop_mod.f90:2759-2767)."""
+    import json
+    import os
+    from abt_amd.beams import PI_R4 as PI4, outbeams_from_pbt_json
+    from tests.golden.make_beam_json import case, groups
+    path = os.path.join(os.path.dirname(__file__), "golden", "beam_export.json")
+    V, F, b, fi = case()
+    got, gfi, ids = outbeams_from_pbt_json(path, V, F, 0.532, with_ids=True)
+    assert got.n == b.n
+    np.testing.assert_array_equal(gfi, fi)
+    np.testing.assert_array_equal(got.ampl, b.ampl)
+    np.testing.assert_array_equal(got.vk7, b.vk7)
+    np.testing.assert_array_equal(got.verts, b.verts)
+    tree = json.load(open(path))["beam tree"]
+    assert sum(1 for t in tree if t["is_int"]) > 0
+    area = lambda f: 0.5 * np.linalg.norm(np.cross(V[F[f - 1][1]] - V[F[f - 1][0]], V[F[f - 1][2]] - V[F[f - 1][0]]))
+    for grp in groups(b.n):
+        a = sum(area(int(fi[i])) for i in grp)
+        want = 0.532 / (2 * np.sqrt(a / PI4)) * 8
+        for i in grp:
+            np.testing.assert_allclose(got.fov[i], want, rtol=1e-14)
+            np.testing.assert_array_equal(got.prop[i], b.prop[grp[0]])
+    assert len(set(ids[[4, 5, 6]])) == 1 and len(set(ids[[20, 21]])) == 1 and ids[3] != ids[4]
