@@ -13,8 +13,12 @@
 #include <math.h>
 #include <stdint.h>
 
+#include <mutex>
+
+#include "../../include/abt_diff.h"
 #include "mueller.h"
 #include "records.h"
+#include "rt_internal.h"
 #include "simpne.h"
 
 namespace abt {
@@ -194,4 +198,75 @@ extern "C" int abt_launch_accumulate(double* total, const double* x, int64_t n, 
 {
     hipLaunchKernelGGL(abt::k_accumulate, dim3(nblk(n)), dim3(abt::MB), 0, st, total, x, n);
     return (int)hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// The epilogue without allocation (abt_finalise_dev_static): its scratch (the total
+// amplitude, 8 doubles per bin, and the theta integrands, 3 per theta) lives in a
+// file-scope device array, one per GPU for the life of the process.  In host code the
+// array's name is not a device address: its address on the current device comes from
+// hipGetSymbolAddress, once per device, and only that address reaches the kernels.
+namespace {
+
+constexpr int64_t STATIC_SCRATCH_DOUBLES = (int64_t)8 << 20;   // 64 MiB
+static __device__ __align__(256) double scratch[STATIC_SCRATCH_DOUBLES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[MAX_DEVICES] = {};
+
+// The scratch array's address on the library's device (looked up on first use there).
+int scratch_on_device(double** out)
+{
+    int rc = abt_ensure_device(nullptr);
+    if (rc) return rc;
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return abt_fail(ABT_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e));
+    if (dev < 0 || dev >= MAX_DEVICES) return abt_fail(ABT_ERR_NODEV, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return abt_fail(ABT_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int64_t abt_finalise_static_max_bins(int64_t n_theta)
+{
+    return n_theta < 0 ? 0 : (STATIC_SCRATCH_DOUBLES - 3 * n_theta) / 8;
+}
+
+extern "C" int abt_finalise_dev_static(const abt_dev_grid* g, double* ab, double* ae, int scaling,
+                                       double ebeam, double eext, double* dev_mueller, double* dev_m1d,
+                                       double* dev_params, void* stream)
+{
+    if (!g || !ab || !ae || !dev_mueller || !dev_m1d || !dev_params)
+        return abt_fail(ABT_ERR_ARG, "abt_finalise_dev_static: null argument");
+    if (g->d.nphi < 3 || g->d.ntheta < 3)
+        return abt_fail(ABT_ERR_ARG, "finalise needs at least 3 theta and 3 phi values (simpne)");
+    if (scaling && !(ebeam > 0 && eext > 0))
+        return abt_fail(ABT_ERR_ARG, "energy scaling needs positive beam and ext energies");
+    const int64_t need = 8 * (int64_t)g->nbins + 3 * (int64_t)g->d.ntheta;
+    if (need > STATIC_SCRATCH_DOUBLES)
+        return abt_fail(ABT_ERR_ARG,
+                        "abt_finalise_dev_static: %lld bins x %d theta need %lld scratch doubles, more than the "
+                        "%lld it holds (at most %lld bins at this theta count); use abt_finalise_dev",
+                        (long long)g->nbins, g->d.ntheta, (long long)need, (long long)STATIC_SCRATCH_DOUBLES,
+                        (long long)abt_finalise_static_max_bins(g->d.ntheta));
+    double* sp = nullptr;
+    int rc = scratch_on_device(&sp);
+    if (rc) return rc;
+    double* tot = sp;
+    double* tmp = sp + 8 * g->nbins;
+    int e = abt_launch_finalise(&g->d, ab, ae, tot, dev_mueller, dev_m1d, tmp, dev_params, scaling, ebeam, eext,
+                                (hipStream_t)stream);
+    if (e) return abt_fail(ABT_ERR_HIP, "finalise launch failed: %s", hipGetErrorString((hipError_t)e));
+    return 0;
 }

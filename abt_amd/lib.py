@@ -35,7 +35,7 @@ EXPORTS = [
     "abt_vertex_record_doubles", "abt_prepare_beams_host", "abt_prepare_beams_dev",
     "abt_grid_create", "abt_grid_destroy", "abt_grid_bins", "abt_far_field_workspace",
     "abt_far_field_dev", "abt_finalise_workspace", "abt_finalise_dev",
-    "abt_far_field_incoherent_dev",
+    "abt_far_field_incoherent_dev", "abt_finalise_dev_static", "abt_finalise_static_max_bins",
     "abt_batch_plan_create", "abt_batch_plan_destroy",
     "abt_batch_workspace", "abt_batch_set_tiles", "abt_batch_get_tiles", "abt_batch_autotune",
     "abt_batch_run", "abt_batch_capture", "abt_batch_launch", "abt_batch_gate_stats", "abt_batch_set_scaling",

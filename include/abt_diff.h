@@ -161,6 +161,21 @@ int abt_far_field_dev(const abt_dev_grid* g, const double* dev_recs, const doubl
                       int64_t nb, int precision, int is_fast_diff, double* dev_out,
                       void* dev_work, size_t work_bytes, void* stream);
 
+/* abt_finalise_dev without any allocation.  No reference counterpart: for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The epilogue's scratch (the total amplitude, 8 doubles per bin, and the
+ * theta integrands, 3 per theta value) is a 64 MiB array resident on each GPU the library
+ * runs on, for the life of the process, instead of a workspace argument or a library
+ * allocation: grids up to abt_finalise_static_max_bins(n_theta) bins (about 1.05e6).
+ * Larger grids are rejected with ABT_ERR_ARG.  Every call on one GPU shares that array:
+ * calls that overlap on one GPU (other streams or threads) race and must be serialised by
+ * the caller.  Arguments and outputs as abt_finalise_dev. */
+int abt_finalise_dev_static(const abt_dev_grid* g, double* dev_ampl_beam, double* dev_ampl_ext,
+                            int scaling, double beam_energy_out, double ext_energy_out,
+                            double* dev_mueller, double* dev_mueller_1d, double* dev_params,
+                            void* stream);
+int64_t abt_finalise_static_max_bins(int64_t n_theta);
+
 /* Conditioning diagnostic (no reference counterpart): dev_intensity[nbins] =
  * sum over beams of the squared Frobenius norm of each beam's FP64 amplitude
  * (the incoherent sum).  sqrt of it over |coherent sum| bounds how much the
