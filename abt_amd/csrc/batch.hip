@@ -105,6 +105,9 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// Occupancy: 4 waves per SIMD (128 VGPRs) for the mixed body and for the restructured FP64
+// body with 8-beam tiles (whose LDS, ~39 KB per workgroup, lets 4 workgroups share a CU; its
+// spills are outside the beam loop), else 3.
 // Register budget: the per-orientation beam-tree result waits in LDS (xbs) while
 // the ext tree is summed, and the phi rotation and integral weights are re-read
 // after the sweeps, so the live state across a sweep is the accumulators and the
@@ -116,7 +119,7 @@ __device__ __forceinline__ double wave_sum(double v)
 // (*a.list_count of them), whose Mueller sums go to mpart at the list position; no
 // integrals (the mixed pass keeps them).
 template <int PREC, int TB, int BPT, int MODE>
-__global__ __launch_bounds__(BB, PREC == 1 ? 4 : 3) void k_batch(BatchArgs a)
+__global__ __launch_bounds__(BB, (PREC == 1 || (PREC == 4 && TB <= 8)) ? 4 : 3) void k_batch(BatchArgs a)
 {
     constexpr bool INC = MODE == BMODE_EST;
     constexpr bool LIST = MODE == BMODE_LIST;

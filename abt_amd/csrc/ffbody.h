@@ -601,15 +601,15 @@ __host__ __device__ __forceinline__ void expi_pi64(double t, const double* __res
     *s = fma(s0, cr, c0 * sr);
 }
 
-// e^{2 pi i t} for an FP64 t (revolutions), from the same table: t = m + j / (2 ETAB_N) + r,
-// |r| <= 1 / (4 ETAB_N), e^{2 pi i t} = tab[j] e^{2 pi i r}
+// e^{2 pi i t} for an FP64 t (revolutions, |t| < 2^20), from the same table: t = J / (2 ETAB_N) + r
+// with J = rint(2 ETAB_N t) and |r| <= 1 / (4 ETAB_N) (exact: t and J / (2 ETAB_N) are multiples of
+// the ulp of t), e^{2 pi i t} = e^{2 pi i J / (2 ETAB_N)} e^{2 pi i r}; the table holds the angle
+// 2 pi J / (2 ETAB_N) modulo 2 pi at index (J + ETAB_N) mod 2 ETAB_N.
 __host__ __device__ __forceinline__ void expi_2pi64(double t, const double* __restrict__ tab, double* c, double* s)
 {
-    const double m = rint(t);
-    const double u = t - m;                                       // [-1/2, 1/2]
-    const double jf = rint((double)(2 * ETAB_N) * u);
-    const double r = fma(-1.0 / (double)(2 * ETAB_N), jf, u);     // |r| <= 1 / (4 ETAB_N)
-    const int j = (int)jf + ETAB_N;
+    const double jf = rint((double)(2 * ETAB_N) * t);
+    const double r = fma(-1.0 / (double)(2 * ETAB_N), jf, t);     // |r| <= 1 / (4 ETAB_N)
+    const int j = ((int)jf + ETAB_N) & (2 * ETAB_N - 1);
     const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
     const double y = (2.0 * PI64) * r;
     const double z = y * y;
@@ -712,6 +712,7 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
         FI = (sb * si - cb * sr) * il;
     } else {
         double ssr = 0.0, ssi = 0.0;
+        bool edges = true;
         if (tri) {
             double t[3], Wr[3], Wi[3];
 #pragma unroll
@@ -719,10 +720,27 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
                 t[l] = fma(qx, r[R_EX + l], fma(qy, r[R_EY + l], kd));
                 expi_2pi64(t[l], etab, &Wr[l], &Wi[l]);
             }
+            // One denominator: the form factor of a triangle is 2 A times the second divided
+            // difference of e^{i alpha} at its vertex phases alpha_l = 2 pi t_l, divided by i^2,
+            //   F = 2 A (W0 d1 + W1 d2 + W2 d0) / ((2 pi)^2 d0 d1 d2),  d_j = t_{j+1} - t_j,
+            // and sum_ref / lambda = -i F / lambda.  Its rounding is that of the edge sum while
+            // every |d_j| >= 1e-3 (relative to the largest edge term, ~ulp / min |d_j|); below,
+            // the edge sum with its product form.
+            const double d0 = t[1] - t[0], d1 = t[2] - t[1], d2 = t[0] - t[2];
+            if (fmin(fabs(d0), fmin(fabs(d1), fabs(d2))) >= 1e-3) {
+                const double nr = fma(Wr[0], d1, fma(Wr[1], d2, Wr[2] * d0));
+                const double ni = fma(Wi[0], d1, fma(Wi[1], d2, Wi[2] * d0));
+                const double f = r[R_AREA] * g.tri_scale * rcp64_1(d0 * d1 * d2);
+                FR = f * ni;
+                FI = -f * nr;
+                edges = false;
+            } else {
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                const int jn = j == 2 ? 0 : j + 1;
-                edge_diff64(Wr[j], Wi[j], Wr[jn], Wi[jn], t[jn] - t[j], qx, qy, r[R_EDX + j], r[R_EDY + j], ssr, ssi);
+                for (int j = 0; j < 3; j++) {
+                    const int jn = j == 2 ? 0 : j + 1;
+                    edge_diff64(Wr[j], Wi[j], Wr[jn], Wi[jn], t[jn] - t[j], qx, qy, r[R_EDX + j], r[R_EDY + j],
+                                ssr, ssi);
+                }
             }
         } else {
             const int nv = (int)r[R_NV];
@@ -743,9 +761,11 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
                 Wip = Win;
             }
         }
-        const double sc = -rcp64_1(6.283185307179586 * lam * q2);
-        FR = ssr * sc;
-        FI = ssi * sc;
+        if (edges) {
+            const double sc = -rcp64_1(6.283185307179586 * lam * q2);
+            FR = ssr * sc;
+            FI = ssi * sc;
+        }
     }
 
     // ---- polarisation (karczewski + rot4, diff_mod.f90:124-198, 630-643) ----

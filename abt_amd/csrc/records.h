@@ -88,6 +88,7 @@ struct DevGrid {
     float inv_lambda_f;                     // 1/lambda (FP32 body of the mixed kernel)
     float inv_krev_f;                       // 1/krev
     double inv_krev;                        // 1/krev (restructured FP64 body)
+    double tri_scale;                       // 1 / (2 pi^2 lambda) (its triangle form factor)
     const double* x;                        // [nbins] far-field bin positions
     const double* y;
     const double* z;
@@ -123,6 +124,7 @@ inline void devgrid_constants(DevGrid& g, double lambda)
     g.inv_lambda_f = (float)(1.0 / lambda);
     g.inv_krev_f = (float)(1.0 / g.krev);
     g.inv_krev = 1.0 / g.krev;
+    g.tri_scale = 1.0 / (2.0 * 9.869604401089358 * lambda);
 }
 
 }  // namespace abt

@@ -41,6 +41,7 @@ PEAK_OF = {"mixed": "mixed", "mixed_gated": "mixed", "fp64": "fp64", "fp64_fast"
 METRIC = "beam·bin diffractions/s mixed-prec vs FP64 at ≤1e-5 Mueller rel err"
 LAM = 0.532
 GATE = {"S11": 1e-6, "elem": 1e-5, "floor": 1e-3}
+TILES = None             # (beams per tile, bins per lane) from --tiles: fixed shape, no autotuning
 
 
 def log(*a):
@@ -122,9 +123,16 @@ class OrientationCase:
                 torch.empty(16 * len(self.th), **f64))
 
     def plan(self, precision, autotune=True):
+        """A batched plan, autotuned on 32 orientations, or with the tile shape of --tiles
+        (beams per tile x bins per lane) where that shape exists for the precision."""
         from abt_amd import lib
         p = lib.BatchPlan(self.g, self.boff, self.eoff, precision)
-        tune = p.autotune(self.rb, self.re, self.vb, self.ve, sample=32) if autotune else {}
+        tune = {}
+        if autotune and TILES and precision != "fp64_accurate":
+            tb, bpt = TILES
+            p.set_tiles(tb, bpt if precision in ("mixed", "mixed_gated") else 1)
+        elif autotune:
+            tune = p.autotune(self.rb, self.re, self.vb, self.ve, sample=32)
         return p, tune
 
     def timed_graph(self, precision, steps, warmup, with_kernel_ms=True):
@@ -293,6 +301,21 @@ def stress_section(nbeam, steps, lam, dev, cpu_sample):
     for prec in ("mixed", "fp64_fast"):
         out[prec]["speedup_vs_fp64"] = out["fp64"]["ms_per_step"] / out[prec]["ms_per_step"]
         out[prec]["accuracy"] = gate_report(mus[prec], mus["fp64"])
+    # the frame-free FP64 model (no kxx/kyy clamps, no split-sum cancellation; a one-orientation
+    # batched plan) as a second reference for all three
+    p = lib.BatchPlan(g, [0, nbeam], [0, 0], "fp64_accurate")
+    mm, ii, m1m = torch.empty(16 * nbins, **f64), torch.empty(6, **f64), torch.empty(16 * len(th), **f64)
+    t = time.perf_counter()
+    p.run(recs, None, mm, ii, m1m, vrecs, None)
+    torch.cuda.synchronize()
+    Mm = mm.cpu().numpy().reshape(16, -1)
+    out["vs_fp64_model"] = {"model": f"batched plan precision fp64_accurate (adjbody.h), one orientation, "
+                                     f"{(time.perf_counter() - t) * 1e3:.0f} ms",
+                            **{prec: gate_report(mus[prec], Mm, th, ph) for prec in ("fp64", "fp64_fast", "mixed")}}
+    p.close()
+    log("  stress against the FP64 model: " + ", ".join(
+        f"{k} floored max {v['floored_elem']['max']:.2e} ({v['failing_bins']} failing bins)"
+        for k, v in out["vs_fp64_model"].items() if k != "model"))
     if cpu_sample > 0:
         import oracle
         cores = min(16, os.cpu_count() or 1)
@@ -360,6 +383,8 @@ def main():
     ap.add_argument("--vis-beams", type=int, default=4096, help="beams of the facet-visibility workload (0: skip)")
     ap.add_argument("--cpu-sample", type=int, default=2,
                     help="orientations of the CPU baseline sample (0: skip; the stress test samples 96 apertures)")
+    ap.add_argument("--tiles", default=None,
+                    help="fixed batched tile shape TBxBPT (e.g. 8x1) instead of autotuning (profiling runs)")
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -367,6 +392,9 @@ def main():
               "one GPU; the multi-GPU path (orientations sharded by beam count, one fixed-order reduction of "
               "the Mueller partials) is a design, not built.", file=sys.stderr)
         sys.exit(2)
+    if args.tiles:
+        global TILES
+        TILES = tuple(int(v) for v in args.tiles.lower().split("x"))
     import torch
     dev = torch.device("cuda:0")
     out = {"metric": METRIC, "unit": "beam·bin/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -587,24 +615,27 @@ def visibility_bench(nbeam, steps, warmup, dev, with_cpu=True):
                         "frac": ach / PEAK["fp64"], "traffic": _traffic("k_vis_main<0,false>"),
                         "kernel": "k_vis_main<0,false>", "kernel_ms": scan_ms, "flop_per_launch": flop}}
     if with_cpu:
-        out["cpu_baseline"] = vis_cpu_baseline(wl, min(nbeam, 1024), value)
+        out["cpu_baseline"] = vis_cpu_baseline(wl, min(nbeam, 4096), value)
     return out
 
 
 def vis_cpu_baseline(wl, nsample, gpu_value):
     """The visibility oracle (oracle/vis_oracle.c, a C restatement of find_vis_int) on the
-    host, one thread, over the first `nsample` beams of the same workload."""
+    host's cores (beams spread over OpenMP threads), over the first `nsample` beams of the
+    same workload."""
     import oracle as O
     topo, v, mid, nz, apn, ino, fin, bmz = wl
-    n = nsample
+    n = min(nsample, len(nz))
+    cores = min(16, os.cpu_count() or 1)
     sub = (topo, v[:n], mid[:n], nz[:n], apn[:n], ino[:n + 1], fin[:ino[n]], bmz[:n])
     t = time.perf_counter()
-    O.find_vis_batch("int", *sub, nthreads=1)
+    O.find_vis_batch("int", *sub, nthreads=cores)
     dt = time.perf_counter() - t
     val = n * topo.nf / dt
-    log(f"visibility cpu baseline: {n} beams in {dt:.2f} s, {val:.4e} facet/s (1 thread)")
-    return {"value": val, "unit": "facet/s", "cores": 1, "kind": "port",
-            "sample": f"the first {n} beams of the same workload, find_vis_int restated in C, 1 thread",
+    log(f"visibility cpu baseline: {n} beams in {dt:.2f} s, {val:.4e} facet/s ({cores} threads)")
+    return {"value": val, "unit": "facet/s", "cores": cores, "kind": "port",
+            "sample": f"the first {n} beams of the same workload, find_vis_int restated in C "
+                      f"(oracle/vis_oracle.c; the reference's tracer is not built here), {cores} threads",
             "gpu_over_cpu": gpu_value / val}
 
 

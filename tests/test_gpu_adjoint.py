@@ -13,8 +13,10 @@ Checks
     apertures keep every bin away from the singular direction of the reference's
     polarisation (see _smooth_vertices);
   * amplitude-matrix gradients against finite differences;
-  * the mixed-precision adjoint against the FP64 adjoint (gradient vectors within 2e-4
-    in norm, relative).
+  * the mixed-precision adjoint against the FP64 adjoint (gradient vectors within 1e-5
+    in norm, relative);
+  * the full adjoint workload (64 orientations, recursion depth 10): both adjoints against
+    directional finite differences on 50 vertices.
 """
 import numpy as np
 import pytest
@@ -103,14 +105,14 @@ def test_adjoint_kernel_matches_cpu_bodies():
                                        atol=1e-8 * np.abs(wa).max())
 
 
-def _retrieval_setup(norient, precision):
+def _retrieval_setup(norient, precision, rec=4):
     import torch
-    case = _particle_case(norient, rec=4)
+    case = _particle_case(norient, rec=rec)
     s = _system(case, precision)
     # target: the particle with vertices perturbed by N(0, 0.05 um), traced again
     rng = np.random.default_rng(77)
     vt = case["verts"] + rng.normal(scale=0.05, size=case["verts"].shape)
-    tt = trace_orientations(vt, case["faces"], case["rots"], rec=4, device=case["dev"])
+    tt = trace_orientations(vt, case["faces"], case["rots"], rec=rec, device=case["dev"])
     st = AG.FarFieldSystem(case["g"], tt["beam_off"].cpu().numpy(), tt["ext_off"].cpu().numpy(),
                            {k: tt["beams"][k] for k in ("prop", "vk7")},
                            {k: tt["ext"][k] for k in ("prop", "vk7")}, precision="fp64_accurate")
@@ -131,7 +133,14 @@ def _loss_of(case, s, V, ab, ae, T11, Tp, w):
 
 def _smooth_vertices(case, s, threshold=1e-5):
     """Particle vertices whose apertures have every bin direction at 1 - |khat.e_y| >
-    threshold (e_y: the aperture frame's in-plane axis, the record's second frame row).
+    threshold (e_y: the aperture frame's in-plane axis, the record's second frame row)."""
+    v = _vertex_margins(case, s)
+    return np.nonzero(np.isfinite(v) & (v > threshold))[0]
+
+
+def _vertex_margins(case, s):
+    """Per particle vertex, the smallest 1 - |khat.e_y| over the bins and the apertures
+    that use it (inf for vertices no aperture uses).
     The reference's polarisation (karczewski, diff_mod.f90:145-181) is singular at
     khat = +-e_y, where m = (e_y - k_y khat) / sqrt(1 - k_y^2) has no limit, and is
     clamped at |k_y| = 0.999999 (diff_mod.f90:151), so within ~0.3 degrees of that axis
@@ -150,8 +159,7 @@ def _smooth_vertices(case, s, threshold=1e-5):
         m = torch.cat([(ey[i:i + 2048] @ X).abs().max(1).values for i in range(0, n, 2048)])
         d = (1 - m)[:, None].expand(n, 3).reshape(-1)
         vmin.scatter_reduce_(0, faces[fi].reshape(-1), d, reduce="amin")
-    v = vmin.cpu().numpy()
-    return np.nonzero(np.isfinite(v) & (v > threshold))[0]
+    return vmin.cpu().numpy()
 
 
 def test_vertex_gradient_matches_finite_differences():
@@ -228,4 +236,77 @@ def test_mixed_adjoint_matches_fp64_adjoint():
         grads.append(V.grad.cpu().numpy().ravel())
     g64, gm = grads
     rel = np.linalg.norm(gm - g64) / np.linalg.norm(g64)
-    assert rel <= 2e-4, rel
+    assert rel <= 1e-5, rel
+
+
+def _directional_fd(case, s, V, ab, ae, T11, Tp, w, vi, d, h=1e-4):
+    """4th- and 2nd-order central differences of the loss along d at vertex vi."""
+    import torch
+    f = {}
+    with torch.no_grad():
+        for m in (-2, -1, 1, 2):
+            Vp = V.detach().clone()
+            Vp[vi] += (m * h) * d
+            f[m] = float(_loss_of(case, s, Vp, ab, ae, T11, Tp, w))
+    return (-f[2] + 8 * f[1] - 8 * f[-1] + f[-2]) / (12 * h), (f[1] - f[-1]) / (2 * h)
+
+
+def test_full_adjoint_workload_matches_finite_differences():
+    """The adjoint workload at its full size: 64 orientations of the rough column traced to
+    recursion depth 10, 361 x 90 bins, the retrieval loss against the particle with
+    vertices perturbed by N(0, 0.05 um).  On 50 random particle vertices, the derivative
+    along a random unit direction by central differences of the FP64 model's loss, step
+    h = 1e-6 um: the FP64 adjoint within 1e-6 of the largest of the 50 (4th-order stencil);
+    the mixed adjoint (FP64 phases, FP32 edge terms and polarisation, FP32 sums over groups
+    of 8 bins) within 2e-5 of it; the two gradient vectors within 1e-5 in norm.
+
+    The step is 1e-6 um, not the 1e-4 um of the smaller check above: at this size every
+    vertex belongs to ~1800 apertures, and each has bins within 1e-6 (in 1 - |khat.e_y|) of
+    the singular axis of the reference's polarisation, inside its clamp |k_y| <= 0.999999
+    (diff_mod.f90:151), where the loss has kinks.  A kink inside the stencil shows as
+    disagreement between the 2nd-order differences with steps h and 2h (their truncation is
+    ~1e-10 here, their rounding ~1e-8 of the largest); such vertices (a few of the 50; the
+    count falls with h) are left out, and at least 40 must remain."""
+    import torch
+    case, s64, T11, Tp, w = _retrieval_setup(64, "fp64_accurate", rec=10)
+    t = case["t"]
+    assert t["beam_off"][-1] > 64 * 2000
+    ab, ae = t["beams"]["ampl"], t["ext"]["ampl"]
+    grads = {}
+    for name, s in (("fp64", s64), ("mixed", _system(case, "mixed"))):
+        V = torch.tensor(case["verts"], dtype=torch.float64, device=case["dev"], requires_grad=True)
+        L = _loss_of(case, s, V, ab, ae, T11, Tp, w)
+        L.backward()
+        grads[name] = V.grad.cpu().numpy()
+    g64, gm = grads["fp64"], grads["mixed"]
+    rel = np.linalg.norm(gm - g64) / np.linalg.norm(g64)
+    assert rel <= 1e-5, rel
+    used = np.nonzero(np.isfinite(_vertex_margins(case, s64)))[0]
+    rng = np.random.default_rng(11)
+    pick = rng.choice(used, 50, replace=False)
+    V = torch.tensor(case["verts"], dtype=torch.float64, device=case["dev"])
+    h = 1e-6
+    fd4, fd2h, fd22h, d64, dm = [], [], [], [], []
+    with torch.no_grad():
+        for vi in pick:
+            d = rng.normal(size=3)
+            d /= np.linalg.norm(d)
+            dt = torch.tensor(d, dtype=torch.float64, device=case["dev"])
+            f = {}
+            for m in (-2, -1, 1, 2):
+                Vp = V.clone()
+                Vp[vi] += (m * h) * dt
+                f[m] = float(_loss_of(case, s64, Vp, ab, ae, T11, Tp, w))
+            fd4.append((-f[2] + 8 * f[1] - 8 * f[-1] + f[-2]) / (12 * h))
+            fd2h.append((f[1] - f[-1]) / (2 * h))
+            fd22h.append((f[2] - f[-2]) / (4 * h))
+            d64.append(g64[vi] @ d)
+            dm.append(gm[vi] @ d)
+    fd4, fd2h, fd22h, d64, dm = map(np.array, (fd4, fd2h, fd22h, d64, dm))
+    scale = np.abs(fd4).max()
+    smooth = np.abs(fd2h - fd22h) <= 1e-6 * scale
+    assert smooth.sum() >= 40, int(smooth.sum())
+    e64 = np.abs(d64 - fd4)[smooth].max() / scale
+    em = np.abs(dm - fd4)[smooth].max() / scale
+    assert e64 <= 1e-6, (e64, em, int(smooth.sum()))
+    assert em <= 2e-5, (e64, em, int(smooth.sum()))
