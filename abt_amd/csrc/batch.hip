@@ -483,7 +483,7 @@ extern "C" const void* abt_batch_kernel(int prec, int tb, int bpt, int mode)
     K(1, 8, 1, 0) K(1, 16, 1, 0) K(1, 32, 1, 0) K(1, 8, 2, 0) K(1, 16, 2, 0) K(1, 32, 2, 0)
     K(1, 8, 1, 1) K(1, 16, 1, 1) K(1, 32, 1, 1) K(1, 8, 2, 1) K(1, 16, 2, 1) K(1, 32, 2, 1)
     K(3, 16, 1, 0)
-    K(4, 8, 1, 0) K(4, 16, 1, 0) K(4, 32, 1, 0)
+    K(4, 8, 1, 0) K(4, 16, 1, 0) K(4, 32, 1, 0) K(4, 8, 2, 0) K(4, 16, 2, 0)
     K(4, 16, 1, 2)
 #undef K
     return nullptr;

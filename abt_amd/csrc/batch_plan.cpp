@@ -65,7 +65,7 @@ int kernel_mode(int precision) { return precision == ABT_MIXED_GATED ? abt::BMOD
 bool shape_allowed(int prec, int tb, int bpt)
 {
     if (prec == ABT_FP64_ACCURATE) return tb == 16 && bpt == 1;
-    if (prec == ABT_FP64_FAST) return bpt == 1;
+    if (prec == ABT_FP64_FAST) return bpt == 1 || tb <= 16;
     return true;
 }
 

@@ -213,14 +213,15 @@ def orientation_section(case, precision, steps, warmup, M64, plain_ms, gate=True
     t = info["tiles"]
     kname = kname % ((t["beams_per_tile"],) if precision == "fp64_fast" else (t["beams_per_tile"], t["bins_per_thread"]))
     out = {"value": value, "unit": "beam·bin/s", "ms_per_step": ms, "steps": steps,
-           "speedup_vs_fp64_plain": plain_ms / ms, **info,
+           "speedup_vs_fp64_plain": plain_ms / ms if np.isfinite(plain_ms) else None, **info,
            "roofline": {"bound": "valu", "achieved": F_ALG * case.bb / (kms * 1e-3) / 1e12, "peak": peak / 1e12,
                         "unit": "TFLOP/s", "frac": F_ALG * case.bb / (kms * 1e-3) / peak,
                         "traffic": _traffic(kname), "kernel": kname, "kernel_ms": kms,
                         "flop_per_beam_bin": F_ALG, "beam_bins_per_launch": case.bb}}
     if M64 is not None:
         out["accuracy"] = gate_report(M, M64, case.th, case.ph)
-    log(f"  {precision}: {ms:.1f} ms/step, {value:.4e} beam.bin/s, {plain_ms / ms:.2f}x the FP64 plain loop"
+    log(f"  {precision}: {ms:.1f} ms/step, {value:.4e} beam.bin/s"
+        + (f", {plain_ms / ms:.2f}x the FP64 plain loop" if np.isfinite(plain_ms) else "")
         + (f", gate {'met' if out['accuracy']['meets_gate'] else 'MISSED'} "
            f"(floored max {out['accuracy']['floored_elem']['max']:.2e}, S11 {out['accuracy']['S11_rel']['max']:.2e})"
            if M64 is not None else "") + (f", refined {info['refined_bins']} bins" if "refined_bins" in info else ""))

@@ -230,7 +230,8 @@ int abt_batch_plan_destroy(abt_batch_plan* plan);
  * 128 chunks; weights, tables). */
 int abt_batch_workspace(const abt_batch_plan* plan, size_t* bytes);
 /* Tile shape: beams staged per LDS tile (8, 16, 32) and bins per lane (1, 2).
- * Default 16 x 1 until set or autotuned (ABT_FP64_ACCURATE: 16 x 1 only). */
+ * Default 16 x 1 until set or autotuned (ABT_FP64_ACCURATE: 16 x 1 only; ABT_FP64_FAST:
+ * 2 bins per lane with 8 or 16 beams per tile only). */
 int abt_batch_set_tiles(abt_batch_plan* plan, int32_t beams_per_tile, int32_t bins_per_thread);
 int abt_batch_get_tiles(const abt_batch_plan* plan, int32_t* beams_per_tile,
                         int32_t* bins_per_thread, int32_t* chunks, int32_t* workgroups);

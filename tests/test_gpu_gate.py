@@ -97,10 +97,10 @@ def test_fp64_fast_batch_meets_gate(case):
     assert_gate(m, ma, np.ones_like(fwd), th, ph)
     _check_params(_params(I), _params(Ia), 1e-9)
     # every tile shape of the plan sums the beams of a bin in the same order
-    for tb in (8, 16, 32):
-        bt.plan.set_tiles(tb, 1)
+    for tb, bpt in ((8, 1), (16, 1), (32, 1), (8, 2), (16, 2)):
+        bt.plan.set_tiles(tb, bpt)
         m2, I2, _ = bt.run()
-        assert np.abs(m2 - m).max() <= 1e-13 * s11.max(), tb
+        assert np.abs(m2 - m).max() <= 1e-13 * s11.max(), (tb, bpt)
 
 
 def test_gated_batch_meets_gate_and_refines_only_flagged_bins():
