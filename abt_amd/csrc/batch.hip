@@ -125,15 +125,17 @@ __global__ __launch_bounds__(BB, (PREC == 1 || (PREC == 4 && TB <= 8)) ? 4 : 3) 
     constexpr bool LIST = MODE == BMODE_LIST;
     constexpr bool PROJ = PREC == 1 || PREC == 4;   // projector sums, finished by mixed_finish
     __shared__ double tile[TB * REC];
-    __shared__ double etab[PREC == 4 ? ETAB_DOUBLES : 1];   // exponential table of the FP64r body
+    __shared__ double etab[PREC == 4 ? ETAB_LDS : 1];   // exponential table and constants of the FP64r body
     __shared__ double xbs[8][BB * BPT];
     __shared__ double red[BB / 64][6];
     __shared__ unsigned s_unit;
     const int tid = threadIdx.x;
     const int nbins = a.g.nbins;
     const double k = a.g.waveno;
-    if (PREC == 4)   // published by the barrier at the top of the unit loop
+    if (PREC == 4) {   // published by the barrier at the top of the unit loop
         for (int q = tid; q < ETAB_DOUBLES; q += BB) etab[q] = a.g.etab[q];
+        if (tid == 0) fp64r_constants(a.g, etab + ETAB_DOUBLES);
+    }
     int ntiles = a.ntiles, nunits = a.nunits, nslots = nbins;
     if (LIST) {
         nslots = *a.list_count;

@@ -55,11 +55,13 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
     // tile).  The FP64 bodies need their registers: one tile, loaded between barriers.
     constexpr bool DB = PREC == 1;
     __shared__ double tile[(DB ? 2 : 1) * TB * REC];
-    __shared__ double etab[PREC == 4 ? ETAB_DOUBLES : 1];   // exponential table of the FP64r body
+    __shared__ double etab[PREC == 4 ? ETAB_LDS : 1];   // exponential table and constants of the FP64r body
     constexpr int PF = DB ? (TB * REC + BLOCK - 1) / BLOCK : 1;   // doubles per lane per tile
     const int tid = threadIdx.x;
-    if (PREC == 4)   // published by the barriers of the tile loop before any use
+    if (PREC == 4) {   // published by the barriers of the tile loop before any use
         for (int q = tid; q < ETAB_DOUBLES; q += BLOCK) etab[q] = g.etab[q];
+        if (tid == 0) fp64r_constants(g, etab + ETAB_DOUBLES);
+    }
     const int bin = blockIdx.x * BLOCK + tid;
     const bool active = bin < g.nbins;
     const int bc = active ? bin : g.nbins - 1;

@@ -582,6 +582,17 @@ inline void fill_exp_table(double* tab)
     }
 }
 
+// The grid constants the FP64r body reads, stored after the table in its LDS block (read
+// there, they take no scalar registers across the beam loop).
+__host__ __device__ __forceinline__ void fp64r_constants(const DevGrid& g, double* c)
+{
+    c[ETAB_KREV - ETAB_DOUBLES] = g.krev;
+    c[ETAB_K2CLAMP - ETAB_DOUBLES] = g.krev_k2clamp;
+    c[ETAB_INV_KREV - ETAB_DOUBLES] = g.inv_krev;
+    c[ETAB_TRI_SCALE - ETAB_DOUBLES] = g.tri_scale;
+    c[ETAB_LAMBDA - ETAB_DOUBLES] = g.lambda;
+}
+
 // e^{i pi t} for an FP64 t (half revolutions): t = 2m + j / ETAB_N + r with |r| <= 1 / (2 ETAB_N)
 // (every step exact), e^{i pi t} = tab[j] e^{i pi r}, the last from its Taylor polynomials
 // (|pi r| <= 3.1e-3: truncation below 2e-18).  tab: the exponential table (in LDS on the device).
@@ -668,9 +679,9 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
     const double qy = fma(-K[3], ux, fma(-K[4], uy, fma(-K[5], uz, r[R_KP2 + 1])));
     const double k1r = r[R_KP2] - qx, k2r = r[R_KP2 + 1] - qy;
     const double rc = fma(bi.x, r[R_COM], fma(bi.y, r[R_COM + 1], bi.z * r[R_COM + 2]));
-    const double kd = g.krev * (fma(-2.0, rc, r[R_CC]) * rcp64_1(fma(s, invd, bi.rad)));   // revolutions
+    const double kd = etab[ETAB_KREV] * (fma(-2.0, rc, r[R_CC]) * rcp64_1(fma(s, invd, bi.rad)));   // revolutions
     const double q2 = fma(qx, qx, qy * qy);
-    const double lam = g.lambda;
+    const double lam = etab[ETAB_LAMBDA];
 
     // ---- aperture integral with the far-field phase: (FR, FI) = e^{i k|r-com|} sum_ref / lambda ----
     double FR, FI;
@@ -730,7 +741,7 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
             if (fmin(fabs(d0), fmin(fabs(d1), fabs(d2))) >= 1e-3) {
                 const double nr = fma(Wr[0], d1, fma(Wr[1], d2, Wr[2] * d0));
                 const double ni = fma(Wi[0], d1, fma(Wi[1], d2, Wi[2] * d0));
-                const double f = r[R_AREA] * g.tri_scale * rcp64_1(d0 * d1 * d2);
+                const double f = r[R_AREA] * etab[ETAB_TRI_SCALE] * rcp64_1(d0 * d1 * d2);
                 FR = f * ni;
                 FI = -f * nr;
                 edges = false;
@@ -769,9 +780,9 @@ __host__ __device__ __forceinline__ void beam_bin_fp64r(const double* __restrict
     }
 
     // ---- polarisation (karczewski + rot4, diff_mod.f90:124-198, 630-643) ----
-    const bool kclamp = fabs(k2r) > g.krev_k2clamp;
+    const bool kclamp = fabs(k2r) > etab[ETAB_K2CLAMP];
     const bool fwd = !(fabs(uz) < FWD_DOT);
-    const double ik = g.inv_krev;
+    const double ik = etab[ETAB_INV_KREV];
     const double k1 = k1r * ik, k2 = k2r * ik;
     const double* RR = r + R_RR;
     const double k3 = fma(RR[6], ux, fma(RR[7], uy, RR[8] * uz));

@@ -105,6 +105,9 @@ struct DevGrid {
 // j = -ETAB_N .. ETAB_N, interleaved, built on the host with the C library.
 constexpr int ETAB_N = 512;
 constexpr int ETAB_DOUBLES = 2 * (2 * ETAB_N + 1);
+// The FP64r body's LDS block: the table, then its grid constants (fp64r_constants).
+constexpr int ETAB_KREV = ETAB_DOUBLES, ETAB_K2CLAMP = ETAB_DOUBLES + 1, ETAB_INV_KREV = ETAB_DOUBLES + 2,
+              ETAB_TRI_SCALE = ETAB_DOUBLES + 3, ETAB_LAMBDA = ETAB_DOUBLES + 4, ETAB_LDS = ETAB_DOUBLES + 5;
 
 // REAL*4 literals of the reference, widened to double
 constexpr double PI_R4 = 3.1415927410125732;

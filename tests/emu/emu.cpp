@@ -18,8 +18,9 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
     memset(&g, 0, sizeof(g));
     g.nphi = nphi; g.ntheta = ntheta; g.nbins = nphi * ntheta; g.theta_i = theta_i;
     devgrid_constants(g, lambda);
-    static double etab[ETAB_DOUBLES];
+    static double etab[ETAB_LDS];
     fill_exp_table(etab);
+    fp64r_constants(g, etab + ETAB_DOUBLES);
     g.etab = etab;
     const int nbins = g.nbins;
 #pragma omp parallel for schedule(dynamic, 64)
