@@ -317,11 +317,14 @@ class BatchPlan:
     beam_off / ext_off: norient+1 offsets of each orientation's records in the
     beam-tree and ext-tree record arrays.  Outputs of run(): mueller (16*nbins,
     summed over orientations), mueller_1d (16*ntheta, optional) and integrals
-    (6*norient: scatt, I2, scatt_beam, I2_beam, scatt_ext, I2_ext; asymmetry = I2/scatt)."""
+    (6*norient: scatt, I2, scatt_beam, I2_beam, scatt_ext, I2_ext; asymmetry = I2/scatt).
+    precision: "fp64_fast" (default; meets the 1e-5 floored Mueller accuracy gate), "fp64"
+    (the reference's formula), "mixed" (FP32 body; faster, misses the gate), "mixed_gated",
+    "fp64_accurate"."""
 
     TILE_SHAPES = [(tb, bpt) for bpt in (1, 2) for tb in (8, 16, 32)]
 
-    def __init__(self, grid: DeviceGrid, beam_off, ext_off, precision="mixed", fast_diff=False):
+    def __init__(self, grid: DeviceGrid, beam_off, ext_off, precision="fp64_fast", fast_diff=False):
         self.grid = grid
         self.beam_off = np.ascontiguousarray(beam_off, dtype=np.int64)
         self.ext_off = np.ascontiguousarray(ext_off, dtype=np.int64)

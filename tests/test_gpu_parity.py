@@ -7,7 +7,11 @@ Tolerances
   exact forward direction of a beam) carry the formula's own ~1e-16/(|q| a)^2
   rounding and are checked at 1e-6 instead.
   Mixed kernel: see check_mixed (amplitude <= 6e-6, normalised Mueller
-  elements |dS_ij| / S11 <= 1e-5 on bright, non-forward bins).
+  elements |dS_ij| / S11 <= 1e-5 on bright, non-forward bins).  These are the
+  mixed kernel's own criteria; they are looser than the accuracy gate
+  (|dS_ij| <= 1e-5 max(|S_ij|, 1e-3 S11), |dS_11| <= 1e-6 S11 on every bin),
+  which the FP32 body does not meet.  The paths that meet the gate are tested
+  against it in test_gpu_gate.py.
 """
 import numpy as np
 import pytest
@@ -161,8 +165,8 @@ def check_mixed(got, ref, fwd):
     (largest for the big external-diffraction apertures at wide angles) and across
     beams raises per-bin amplitude errors to a few 1e-6:
       amplitude error <= 6e-6 (p99.9 <= 4e-6), S11 relative error <= 1e-5
-      (p99.9 <= 6e-6), and the accuracy gate on normalised Mueller elements,
-      |dS_ij| / S11 <= 1e-5."""
+      (p99.9 <= 6e-6), and normalised Mueller elements |dS_ij| / S11 <= 1e-5 (not the
+      floored accuracy gate, which this body misses)."""
     import oracle as O_
     err = ampl_rel_err(got, ref)
     amp = np.sqrt((abs(ref) ** 2).sum(axis=(0, 1)))
