@@ -10,7 +10,8 @@
 ! geometry_type records (types_mod.f90:8-17, :156-190) into the C structures.
 !
 ! Precision: the environment variable ABT_PRECISION selects "fp64" (default,
-! the reference's formulas in FP64) or "mixed" (FP32 body, FP64 phases).
+! the reference's formulas in FP64), "fp64_fast" (the exact form factor in a
+! restructured FP64 body) or "mixed" (FP32 body, FP64 phases).
 ! Errors from the library stop the program with its message, as the reference
 ! stops on fatal conditions.
 module diff_mod
@@ -98,7 +99,9 @@ module diff_mod
         integer :: length, status
         call get_environment_variable("ABT_PRECISION", val, length, status)
         precision_flag = 0
-        if (status == 0 .and. val(1:5) == 'mixed') precision_flag = 1
+        if (status /= 0) return
+        if (val(1:5) == 'mixed') precision_flag = 1
+        if (val(1:9) == 'fp64_fast') precision_flag = 3
     end function
 
     ! Flatten a beam tree: amplitude matrices, e-perp, propagation vectors,

@@ -26,7 +26,7 @@ def _case():
     return V, f, b.subset(ib), bfi[ib], e.subset(ie), efi[ie]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fast", "mixed"])
 def test_fortran_drop_in_matches_oracle(orc, precision):
     th, ph = grid((0, 5, 180), (0, 10, 350))
     V, faces, b, bfi, e, efi = _case()
@@ -43,6 +43,12 @@ def test_fortran_drop_in_matches_oracle(orc, precision):
         _check_fp64(out["ampl_far_beam"], rb, fb)
         _check_fp64(out["ampl_far_ext_diff"], re, fe)
         _check_fp64(out["amplc"], r1, _forward_mask(th, ph, [b.subset([0])]))
+    elif precision == "fp64_fast":
+        # the exact form factor: within 1e-7 of the reference formula off the exact forward
+        # directions, where the formula's own rounding sets the level
+        for got, ref, fwd in ((out["ampl_far_beam"], rb, fb), (out["ampl_far_ext_diff"], re, fe)):
+            err = ampl_rel_err(got, ref)
+            assert err[~fwd].max() <= 1e-7, summary(err[~fwd])
     else:
         check_mixed(out["ampl_far_beam"], rb, fb)
         check_mixed(out["ampl_far_ext_diff"], re, fe)
