@@ -389,9 +389,9 @@ def main():
     ap.add_argument("--json-out", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): every GPU box of this build has "
-              "one GPU; the multi-GPU path (orientations sharded by beam count, one fixed-order reduction of "
-              "the Mueller partials) is a design, not built.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the multi-GPU path "
+              "(orientations sharded by beam count, one fixed-order reduction of the Mueller partials) "
+              "is a design, not built.", file=sys.stderr)
         sys.exit(2)
     if args.tiles:
         global TILES
