@@ -161,6 +161,11 @@ def test_fp64_quadrilateral_apertures(orc):
     # the mixed kernel's polygon path (vertex exponentials from the vertex records)
     mb, _ = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision="mixed")
     check_mixed(mb, rb, _forward_mask(th, ph, [b]))
+    # the restructured FP64 body's polygon path (edge sum over the vertex records)
+    fb, _ = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision="fp64_fast")
+    fwd = _forward_mask(th, ph, [b])
+    err = ampl_rel_err(fb, rb)
+    assert err[~fwd].max() <= 1e-7, summary(err[~fwd])
 
 
 def test_empty_trees_give_zeros():
@@ -172,7 +177,7 @@ def test_empty_trees_give_zeros():
 def test_deterministic_repeat():
     th, ph = grid((0, 1, 180), (0, 4, 356))
     b = random_triangles(300, seed=4, box=40.0)
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "fp64_fast", "mixed"):
         a1, _ = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision=prec, raw=True)
         a2, _ = lib.diff_main(th, ph, LAM, b, BeamSet.empty(), precision=prec, raw=True)
         np.testing.assert_array_equal(a1, a2)
