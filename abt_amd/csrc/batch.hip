@@ -83,7 +83,7 @@ __device__ __forceinline__ void sweep(const BatchArgs& a, const double* __restri
 #pragma unroll
                 for (int j = 0; j < BPT; j++) {
                     if (PREC == 0)
-                        beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], cp[j], sp[j], a.fast_diff, acc[j]);
+                        beam_bin_fp64(tile + bb * REC, vrecs, a.g, bi[j], etab, a.fast_diff, acc[j]);
                     else if (PREC == 4)
                         beam_bin_fp64r(tile + bb * REC, vrecs, a.g, bi[j], etab, a.fast_diff, acc[j]);
                     else {
@@ -123,18 +123,19 @@ __global__ __launch_bounds__(BB, (PREC == 1 || (PREC == 4 && TB <= 8)) ? 4 : 3) 
 {
     constexpr bool INC = MODE == BMODE_EST;
     constexpr bool LIST = MODE == BMODE_LIST;
-    constexpr bool PROJ = PREC == 1 || PREC == 4;   // projector sums, finished by mixed_finish
+    constexpr bool ROT = PREC == 0 || PREC == 1 || PREC == 4;   // temp_rot1 applied after the beam sum
     __shared__ double tile[TB * REC];
-    __shared__ double etab[PREC == 4 ? ETAB_LDS : 1];   // exponential table and constants of the FP64r body
+    // the exponential table and the FP64r body's constants, or the table with its low parts
+    __shared__ double etab[PREC == 4 ? ETAB_LDS : (PREC == 0 ? ETAB_GRID : 1)];
     __shared__ double xbs[8][BB * BPT];
     __shared__ double red[BB / 64][6];
     __shared__ unsigned s_unit;
     const int tid = threadIdx.x;
     const int nbins = a.g.nbins;
     const double k = a.g.waveno;
-    if (PREC == 4) {   // published by the barrier at the top of the unit loop
-        for (int q = tid; q < ETAB_DOUBLES; q += BB) etab[q] = a.g.etab[q];
-        if (tid == 0) fp64r_constants(a.g, etab + ETAB_DOUBLES);
+    if (PREC == 0 || PREC == 4) {   // published by the barrier at the top of the unit loop
+        for (int q = tid; q < (PREC == 4 ? ETAB_DOUBLES : ETAB_GRID); q += BB) etab[q] = a.g.etab[q];
+        if (PREC == 4 && tid == 0) fp64r_constants(a.g, etab + ETAB_DOUBLES);
     }
     int ntiles = a.ntiles, nunits = a.nunits, nslots = nbins;
     if (LIST) {
@@ -165,8 +166,8 @@ __global__ __launch_bounds__(BB, (PREC == 1 || (PREC == 4 && TB <= 8)) ? 4 : 3) 
             const int bt = iphi + a.g.nphi * a.g.theta_i;
             bi[j].xt = a.g.x[bt]; bi[j].yt = a.g.y[bt]; bi[j].zt = a.g.z[bt];
             bi[j].rad = (PREC == 3 || PREC == 4) ? a.g.rad[bc] : 0.0;
-            cp[j] = PROJ ? 0.0 : a.g.cphi[iphi];   // the projector bodies rotate after the sum
-            sp[j] = PROJ ? 0.0 : a.g.sphi[iphi];
+            cp[j] = ROT ? 0.0 : a.g.cphi[iphi];   // the other bodies rotate after the sum
+            sp[j] = ROT ? 0.0 : a.g.sphi[iphi];
         }
         const int o0 = a.chunk_o[c], o1 = a.chunk_o[c + 1];
         double* mp = a.mpart + (int64_t)c * 16 * nbins;
@@ -190,10 +191,11 @@ __global__ __launch_bounds__(BB, (PREC == 1 || (PREC == 4 && TB <= 8)) ? 4 : 3) 
                 if (seg == 1) break;
 #pragma unroll
                 for (int j = 0; j < BPT; j++) {
-                    if (PROJ) {
+                    if (ROT) {
                         const int iphi = bins[j] % a.g.nphi;
                         if (PREC == 4) fp64r_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi], a.g.kr_rev[bins[j]]);
-                        else mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
+                        else if (PREC == 1) mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
+                        else apply_temp_rot1(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                     }
                     if (PREC == 3) model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bins[j]]);
 #pragma unroll
@@ -206,10 +208,11 @@ __global__ __launch_bounds__(BB, (PREC == 1 || (PREC == 4 && TB <= 8)) ? 4 : 3) 
             for (int j = 0; j < BPT; j++) {
                 const int slot = slot0 + j * BB;
                 const int bc = bins[j];
-                if (PROJ) {
+                if (ROT) {
                     const int iphi = bc % a.g.nphi;
                     if (PREC == 4) fp64r_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi], a.g.kr_rev[bc]);
-                    else mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
+                    else if (PREC == 1) mixed_finish(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
+                    else apply_temp_rot1(acc[j], a.g.cphi[iphi], a.g.sphi[iphi]);
                 }
                 if (PREC == 3) model_finish(acc[j], cp[j], sp[j], a.g.kr_rev[bc]);
                 double xb[8], xe[8], xt[8], m[16];

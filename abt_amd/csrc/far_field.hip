@@ -9,9 +9,10 @@
 // workgroups (a small last round); chunk partials are then summed in chunk order
 // by k_reduce_chunks.
 //
-// k_ff_fp64  -- the reference's formulas in FP64, operation for operation
-//               (diffraction, diff_mod.f90:576-670; karczewski :124-198;
-//               contour_integral :357-421), FMA contraction off.
+// k_ff_fp64  -- the reference's formulas in FP64 (diffraction, diff_mod.f90:576-670;
+//               karczewski :124-198; contour_integral :357-421), every phase formed with
+//               the reference's operations (bit-identical arguments of every sine and
+//               cosine), the amplitude algebra around them reorganised (beam_bin_fp64).
 // k_ff_fp64r -- (PREC 4) the mixed kernel's algorithm below with every step in FP64.
 // k_ff_mixed -- the same quantity restructured for the vector ALUs:
 //               * geometry, phases and their reduction to octants in FP64
@@ -55,12 +56,13 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
     // tile).  The FP64 bodies need their registers: one tile, loaded between barriers.
     constexpr bool DB = PREC == 1;
     __shared__ double tile[(DB ? 2 : 1) * TB * REC];
-    __shared__ double etab[PREC == 4 ? ETAB_LDS : 1];   // exponential table and constants of the FP64r body
+    // the exponential table and the FP64r body's constants, or the table with its low parts
+    __shared__ double etab[PREC == 4 ? ETAB_LDS : (PREC != 1 ? ETAB_GRID : 1)];
     constexpr int PF = DB ? (TB * REC + BLOCK - 1) / BLOCK : 1;   // doubles per lane per tile
     const int tid = threadIdx.x;
-    if (PREC == 4) {   // published by the barriers of the tile loop before any use
-        for (int q = tid; q < ETAB_DOUBLES; q += BLOCK) etab[q] = g.etab[q];
-        if (tid == 0) fp64r_constants(g, etab + ETAB_DOUBLES);
+    if (PREC != 1) {   // published by the barriers of the tile loop before any use
+        for (int q = tid; q < (PREC == 4 ? ETAB_DOUBLES : ETAB_GRID); q += BLOCK) etab[q] = g.etab[q];
+        if (PREC == 4 && tid == 0) fp64r_constants(g, etab + ETAB_DOUBLES);
     }
     const int bin = blockIdx.x * BLOCK + tid;
     const bool active = bin < g.nbins;
@@ -123,7 +125,7 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
                 for (int bb = 0; bb < nt; bb++) beam_bin_fp64r(cur + bb * REC, vrecs, g, bi, etab, fast_diff, acc);
             } else {
                 for (int bb = 0; bb < nt; bb++)
-                    beam_bin_fp64(cur + bb * REC, vrecs, g, bi, cp, sp, fast_diff, acc, PREC == 2);
+                    beam_bin_fp64(cur + bb * REC, vrecs, g, bi, etab, fast_diff, acc, PREC == 2);
             }
         }
         if (DB) {
@@ -138,6 +140,7 @@ void k_far_field(DevGrid g, const double* __restrict__ recs, const double* __res
         }
     }
     if (!active) return;
+    if (PREC == 0) apply_temp_rot1(acc, cp, sp);
     if (PREC == 1) mixed_finish(acc, cp, sp);
     if (PREC == 4) fp64r_finish(acc, cp, sp, g.kr_rev[bin]);
     const int nbins = g.nbins;

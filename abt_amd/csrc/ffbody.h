@@ -38,130 +38,6 @@ __host__ __device__ __forceinline__ float abt_rcpf(float x)
 }
 
 // ---------------------------------------------------------------------------
-// FP64 plain kernel body: one beam, one bin.  acc: re/im of (1,1) (2,1) (1,2) (2,2)
-__host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict__ r,
-                                              const double* __restrict__ vrecs,
-                                              const DevGrid& g, const BinIn& bi, double cp,
-                                              double sp, int fast_diff, double acc[8],
-                                              bool incoherent = false)
-{
-#pragma clang fp contract(off)
-    const double x1 = bi.x - r[R_COM], y1 = bi.y - r[R_COM + 1], z1 = bi.z - r[R_COM + 2];
-    const double r1 = sqrt(x1 * x1 + y1 * y1 + z1 * z1);
-    if (fast_diff) {
-        double vx = x1 / r1, vy = y1 / r1, vz = z1 / r1;
-        if (vx * r[R_PROP0] + vy * r[R_PROP0 + 1] + vz * r[R_PROP0 + 2] < r[R_COSFOV]) return;
-    }
-    const double* R = r + R_RR;
-    const double x3 = R[0] * x1 + R[1] * y1 + R[2] * z1;
-    const double y3 = R[3] * x1 + R[4] * y1 + R[5] * z1;
-    const double z3 = R[6] * x1 + R[7] * y1 + R[8] * z1;
-    // karczewski (diff_mod.f90:145-181)
-    double k1 = x3 / r1, k2 = y3 / r1, k3 = z3 / r1;
-    if (fabs(k2) > K2CLAMP) k2 = copysign(K2CLAMP, k2);
-    const double Kx = r[R_PROP2], Ky = r[R_PROP2 + 1], Kz = r[R_PROP2 + 2];
-    const double sq = sqrt(1 - k2 * k2);
-    const double m1 = -(k1 * k2 / sq), m2 = sq, m3 = -(k2 * k3 / sq);
-    const double frac = sqrt((1 - k2 * k2) / (1 - Ky * Ky));
-    const double a1m = -(Kz * frac);
-    const double b2m = -(k3 / frac);
-    const double b1m = -(k1 * k2 / frac) + Kx * Ky * frac;
-    const double d11 = 0.5 * (a1m + b2m), d12 = 0.5 * b1m, d21 = 0.5 * (-b1m),
-                 d22 = 0.5 * (b2m + a1m);
-    // scattering-plane vector hc (diff_mod.f90:630-635)
-    const double i1 = r[R_INC2], i2 = r[R_INC2 + 1], i3 = r[R_INC2 + 2];
-    double kk1 = k1, kk2 = k2, kk3 = k3;
-    if (!(fabs(i1 * k1 + i2 * k2 + i3 * k3) < FWD_DOT)) {
-        const double xt1 = bi.xt - r[R_COM], yt1 = bi.yt - r[R_COM + 1], zt1 = bi.zt - r[R_COM + 2];
-        const double rt = sqrt(xt1 * xt1 + yt1 * yt1 + zt1 * zt1);
-        kk1 = (R[0] * xt1 + R[1] * yt1 + R[2] * zt1) / rt;
-        kk2 = (R[3] * xt1 + R[4] * yt1 + R[5] * zt1) / rt;
-        kk3 = (R[6] * xt1 + R[7] * yt1 + R[8] * zt1) / rt;
-    }
-    double h1 = i2 * kk3 - i3 * kk2, h2 = i3 * kk1 - i1 * kk3, h3 = i1 * kk2 - i2 * kk1;
-    const double nh = sqrt(h1 * h1 + h2 * h2 + h3 * h3);
-    h1 = h1 / nh; h2 = h2 / nh; h3 = h3 / nh;
-    double e1 = k2 * m3 - k3 * m2, e2 = k3 * m1 - k1 * m3, e3 = k1 * m2 - k2 * m1;
-    const double ne = sqrt(e1 * e1 + e2 * e2 + e3 * e3);
-    e1 = e1 / ne; e2 = e2 / ne; e3 = e3 / ne;
-    const double p = h1 * m1 + h2 * m2 + h3 * m3;     // rot4(1,1) = rot4(2,2)
-    const double qh = h1 * e1 + h2 * e2 + h3 * e3;    // rot4(2,1); rot4(1,2) = -qh
-    // amplC = rot4 . (diff_ampl . (ampl . temp_rot1)), temp_rot1 = [[-s,-c],[c,-s]]
-    const double* A = r + R_AMPL;
-    const double a11r = A[0], a11i = A[1], a21r = A[2], a21i = A[3];
-    const double a12r = A[4], a12i = A[5], a22r = A[6], a22i = A[7];
-    const double t11 = -sp, t12 = -cp, t21 = cp, t22 = -sp;
-    const double at11r = a11r * t11 + a12r * t21, at11i = a11i * t11 + a12i * t21;
-    const double at12r = a11r * t12 + a12r * t22, at12i = a11i * t12 + a12i * t22;
-    const double at21r = a21r * t11 + a22r * t21, at21i = a21i * t11 + a22i * t21;
-    const double at22r = a21r * t12 + a22r * t22, at22i = a21i * t12 + a22i * t22;
-    const double dt11r = d11 * at11r + d12 * at21r, dt11i = d11 * at11i + d12 * at21i;
-    const double dt12r = d11 * at12r + d12 * at22r, dt12i = d11 * at12i + d12 * at22i;
-    const double dt21r = d21 * at11r + d22 * at21r, dt21i = d21 * at11i + d22 * at21i;
-    const double dt22r = d21 * at12r + d22 * at22r, dt22i = d21 * at12i + d22 * at22i;
-    const double mq = -qh;
-    const double c11r = p * dt11r + mq * dt21r, c11i = p * dt11i + mq * dt21i;
-    const double c12r = p * dt12r + mq * dt22r, c12i = p * dt12i + mq * dt22i;
-    const double c21r = qh * dt11r + p * dt21r, c21i = qh * dt11i + p * dt21i;
-    const double c22r = qh * dt12r + p * dt22r, c22i = qh * dt12i + p * dt22i;
-    // contour_integral (diff_mod.f90:357-421) for this bin
-    const double w = g.waveno, lam = g.lambda;
-    const double bvk = w * sqrt(x3 * x3 + y3 * y3 + z3 * z3);
-    double kxx = Kx * w - w * w * x3 / bvk;
-    double kyy = Ky * w - w * w * y3 / bvk;
-    if (fabs(kxx) < KMIN) kxx = KMIN;
-    if (fabs(kyy) < KMIN) kyy = KMIN;
-    double sb, cb;
-    sincos_f64(bvk, &sb, &cb);
-    double ar = 0.0, ai = 0.0;
-    const auto edge = [&](double xj, double yj, double mj, double nj, double dx, double dy) {
-#pragma clang fp contract(off)
-        const double delta = kxx * xj + kyy * yj;
-        const double delta1 = kyy * mj + kxx;
-        const double delta2 = kxx * nj + kyy;
-        const double omega1 = dx * delta1;
-        const double omega2 = dy * delta2;
-        const double alpha = 1 / (2 * kyy * delta1);
-        const double beta = 1 / (2 * kxx * delta2);
-        double s0, c0, s1, c1, s2, c2;
-        sincos_f64(delta, &s0, &c0);
-        sincos_f64(delta + omega1, &s1, &c1);
-        sincos_f64(delta + omega2, &s2, &c2);
-        const double sumim = alpha * (c0 - c1) - beta * (c0 - c2);
-        const double sumre = -(alpha * (s0 - s1)) + beta * (s0 - s2);
-        ar = ar + (cb * sumre - sb * sumim) / lam;
-        ai = ai + (cb * sumim + sb * sumre) / lam;
-    };
-    // triangles: edges inline in the record; polygons: the vertex records.  Two
-    // loops, so each reads one address space (records staged in LDS, vrecs global).
-    const int nv = (int)r[R_NV];
-    if (nv <= 3) {
-        for (int j = 0; j < nv; j++)
-            edge(r[R_EX + j], r[R_EY + j], r[R_EM + j], r[R_EN + j], r[R_EDX + j], r[R_EDY + j]);
-    } else {
-        const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
-        for (int j = 0; j < nv; j++) {
-            const double* v = vr + VREC * j;
-            edge(v[V_X], v[V_Y], v[V_M], v[V_N], v[V_DX], v[V_DY]);
-        }
-    }
-    const double o0 = c11r * ar - c11i * ai, o1 = c11r * ai + c11i * ar;
-    const double o2 = c21r * ar - c21i * ai, o3 = c21r * ai + c21i * ar;
-    const double o4 = c12r * ar - c12i * ai, o5 = c12r * ai + c12i * ar;
-    const double o6 = c22r * ar - c22i * ai, o7 = c22r * ai + c22i * ar;
-    if (incoherent) {
-        // diagnostic: sum over beams of |amplC|_F^2 (conditioning of the coherent sum)
-        acc[0] = acc[0] + ((((o0 * o0 + o1 * o1) + (o2 * o2 + o3 * o3)) + (o4 * o4 + o5 * o5)) +
-                           (o6 * o6 + o7 * o7));
-        return;
-    }
-    acc[0] = acc[0] + o0; acc[1] = acc[1] + o1;
-    acc[2] = acc[2] + o2; acc[3] = acc[3] + o3;
-    acc[4] = acc[4] + o4; acc[5] = acc[5] + o5;
-    acc[6] = acc[6] + o6; acc[7] = acc[7] + o7;
-}
-
-// ---------------------------------------------------------------------------
 // Mixed-precision kernel body.
 
 // sin/cos(2 pi t) for an FP64 revolution count t: quadrant reduction in FP64,
@@ -572,13 +448,20 @@ __host__ __device__ __forceinline__ double rsq64_1(double x)
     return fma(0.5 * y, fma(-x * y, y, 1.0), y);
 }
 
-// The table of e^{i pi j / ETAB_N} (DevGrid::etab), built by the host.
+// The table of e^{i pi j / ETAB_N} (DevGrid::etab, ETAB_GRID doubles), built by the host: the
+// angle and its cosine and sine in x87 extended precision (64-bit significands), rounded once to
+// double, and the remainders (the low parts, at ETAB_LO).
 inline void fill_exp_table(double* tab)
 {
+    const long double pi_l = 3.14159265358979323846264338327950288L;
     for (int j = -ETAB_N; j <= ETAB_N; j++) {
-        const double a = PI64 * (double)j / (double)ETAB_N;
-        tab[2 * (j + ETAB_N)] = cos(a);
-        tab[2 * (j + ETAB_N) + 1] = sin(a);
+        const long double a = pi_l * (long double)j / (long double)ETAB_N;
+        const long double c = cosl(a), s = sinl(a);
+        const int i = 2 * (j + ETAB_N);
+        tab[i] = (double)c;
+        tab[i + 1] = (double)s;
+        tab[ETAB_LO + i] = (double)(c - (long double)tab[i]);
+        tab[ETAB_LO + i + 1] = (double)(s - (long double)tab[i + 1]);
     }
 }
 
@@ -628,6 +511,185 @@ __host__ __device__ __forceinline__ void expi_2pi64(double t, const double* __re
     const double cr = fma(z, fma(z, 1.0 / 24.0, -0.5), 1.0);
     *c = fma(c0, cr, -s0 * sr);
     *s = fma(s0, cr, c0 * sr);
+}
+
+// sin and cos of an FP64 angle x (radians, |x| < 2^40) from the same table with its low parts:
+// x = n pi / ETAB_N + r with n = rint(x ETAB_N / pi), formed in the low bits of a 1.5 * 2^52
+// shift, and r by a two-part Cody-Waite reduction (|r| <= pi / (2 ETAB_N) + 2^-30, error below
+// 1e-18); e^{ix} = tab[n mod 2 ETAB_N] e^{ir}, the last from its Taylor polynomials (truncation
+// below 2e-19 relative), the table entry taken as a double-double and the correction summed
+// before its high part: ~0.5 ulp, the result rounded nearly correctly.  tab: ETAB_GRID doubles.
+constexpr double SINCOS_SHIFT = 6755399441055744.0;                   // 1.5 * 2^52
+constexpr double N_OVER_PI = (double)ETAB_N / 3.141592653589793;
+constexpr double PI_OVER_N_HI = 3.141592653589793 / (double)ETAB_N;   // exact: a power-of-two scaling
+constexpr double PI_OVER_N_LO = 1.2246467991473532e-16 / (double)ETAB_N;
+__host__ __device__ __forceinline__ void sincos_tab64(double x, const double* __restrict__ tab, double* s,
+                                                      double* c)
+{
+    const double t = fma(x, N_OVER_PI, SINCOS_SHIFT);
+    const double n = t - SINCOS_SHIFT;
+    double r = fma(-n, PI_OVER_N_HI, x);
+    r = fma(-n, PI_OVER_N_LO, r);
+    const int j = ((int)(uint32_t)__builtin_bit_cast(uint64_t, t) + ETAB_N) & (2 * ETAB_N - 1);
+    const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
+    const double cl = tab[ETAB_LO + 2 * j], sl = tab[ETAB_LO + 2 * j + 1];
+    const double z = r * r;
+    const double sr = fma(r * z, fma(z, 1.0 / 120.0, -1.0 / 6.0), r);
+    const double cm1 = z * fma(z, 1.0 / 24.0, -0.5);                   // cos r - 1
+    *s = s0 + fma(c0, sr, fma(s0, cm1, sl));
+    *c = c0 + fma(-s0, sr, fma(c0, cm1, cl));
+}
+
+// a / b given y = 1 / b correctly rounded: q = a y, r = a - b q (exact), q + r y.  Equal to the
+// IEEE quotient (Markstein's correction; no overflow or underflow in the kernels' ranges), at
+// three operations per quotient once y is known.
+__host__ __device__ __forceinline__ double div_by(double a, double b, double y)
+{
+    const double q = a * y;
+    return fma(fma(-b, q, a), y, q);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 plain kernel body: one beam, one bin (diffraction, diff_mod.f90:576-670; karczewski
+// :124-198; contour_integral :357-421).  Every input of a sine or cosine -- the bin offset
+// and its rotation into the aperture frame, bvk = k |x3|, kxx and kyy with their clamps, and per
+// edge delta, delta1, delta2, omega1, omega2 and the sums delta + omega -- is formed with the
+// reference's operations in its order, without contraction, so the phases are the reference's
+// doubles bit for bit (the quotients through div_by equal the divisions).  The amplitude algebra
+// around them is reorganised; each step there changes a result by a few ulp at most:
+//   * sines and cosines from the table (sincos_tab64);
+//   * the Karczewski direction normalised by reciprocal square roots, the ratio
+//     sqrt((1 - k_y^2) / (1 - K_y^2)) from the record's 1 / sqrt(1 - K_y^2);
+//   * rot4 . diff_ampl = g I + h J (J = [[0,1],[-1,0]]: both factors have that form), applied
+//     to the amplitude matrix; temp_rot1 depends on phi only and is applied to the bin's sum
+//     after the beam loop (apply_temp_rot1).
+// The edge terms of contour_integral cancel one another, so they keep the reference's
+// operations as well (the divisions included), in its order.
+// acc: re/im of (1,1) (2,1) (1,2) (2,2), before temp_rot1.  tab: the exponential table (LDS).
+// incoherent: acc[0] += |amplC|_F^2 instead (temp_rot1 is orthogonal: the norm is final).
+__host__ __device__ __forceinline__ void beam_bin_fp64(const double* __restrict__ r,
+                                                       const double* __restrict__ vrecs,
+                                                       const DevGrid& g, const BinIn& bi,
+                                                       const double* __restrict__ tab, int fast_diff,
+                                                       double acc[8], bool incoherent = false)
+{
+#pragma clang fp contract(off)
+    const double x1 = bi.x - r[R_COM], y1 = bi.y - r[R_COM + 1], z1 = bi.z - r[R_COM + 2];
+    const double r1 = sqrt(x1 * x1 + y1 * y1 + z1 * z1);
+    if (fast_diff) {
+        double vx = x1 / r1, vy = y1 / r1, vz = z1 / r1;
+        if (vx * r[R_PROP0] + vy * r[R_PROP0 + 1] + vz * r[R_PROP0 + 2] < r[R_COSFOV]) return;
+    }
+    const double* R = r + R_RR;
+    const double x3 = R[0] * x1 + R[1] * y1 + R[2] * z1;
+    const double y3 = R[3] * x1 + R[4] * y1 + R[5] * z1;
+    const double z3 = R[6] * x1 + R[7] * y1 + R[8] * z1;
+    const double Kx = r[R_PROP2], Ky = r[R_PROP2 + 1], Kz = r[R_PROP2 + 2];
+    // the phase inputs of contour_integral (diff_mod.f90:357-375)
+    const double w = g.waveno;
+    const double bvk = w * sqrt(x3 * x3 + y3 * y3 + z3 * z3);
+    const double ibvk = 1.0 / bvk;
+    const double ww = w * w;
+    double kxx = Kx * w - div_by(ww * x3, bvk, ibvk);
+    double kyy = Ky * w - div_by(ww * y3, bvk, ibvk);
+    if (fabs(kxx) < KMIN) kxx = KMIN;
+    if (fabs(kyy) < KMIN) kyy = KMIN;
+    // karczewski (diff_mod.f90:145-181): k = x3 / r1 as the reference divides, since k also
+    // decides the k_y clamp and the forward test below
+    const double ir1 = 1.0 / r1;
+    const double k1 = div_by(x3, r1, ir1), k3 = div_by(z3, r1, ir1);
+    double k2 = div_by(y3, r1, ir1);
+    if (fabs(k2) > K2CLAMP) k2 = copysign(K2CLAMP, k2);
+    const double omk = fma(-k2, k2, 1.0);
+    const double isq = rsq64(omk);                       // 1 / sqrt(1 - k2^2)
+    const double k12 = k1 * k2;
+    const double m1 = -(k12 * isq), m2 = omk * isq, m3 = -(k2 * k3 * isq);
+    const double frac = m2 * r[R_FRAC];                  // sqrt((1 - k2^2) / (1 - Ky^2))
+    const double ifrac = isq * r[R_ISIG];                // its reciprocal
+    const double a1m = -(Kz * frac), b2m = -(k3 * ifrac);
+    const double b1m = fma(Kx * Ky, frac, -(k12 * ifrac));
+    const double d11 = 0.5 * (a1m + b2m), d12 = 0.5 * b1m;   // diff_ampl = d11 I + d12 J
+    // scattering-plane vector hc (diff_mod.f90:630-635) and rot4 = p I - qh J
+    const double i1 = r[R_INC2], i2 = r[R_INC2 + 1], i3 = r[R_INC2 + 2];
+    double kk1 = k1, kk2 = k2, kk3 = k3;
+    if (!(fabs(i1 * k1 + i2 * k2 + i3 * k3) < FWD_DOT)) {
+        const double xt1 = bi.xt - r[R_COM], yt1 = bi.yt - r[R_COM + 1], zt1 = bi.zt - r[R_COM + 2];
+        const double irt = 1.0 / sqrt(xt1 * xt1 + yt1 * yt1 + zt1 * zt1);
+        kk1 = (R[0] * xt1 + R[1] * yt1 + R[2] * zt1) * irt;
+        kk2 = (R[3] * xt1 + R[4] * yt1 + R[5] * zt1) * irt;
+        kk3 = (R[6] * xt1 + R[7] * yt1 + R[8] * zt1) * irt;
+    }
+    const double h1 = fma(i2, kk3, -(i3 * kk2)), h2 = fma(i3, kk1, -(i1 * kk3)), h3 = fma(i1, kk2, -(i2 * kk1));
+    const double e1 = fma(k2, m3, -(k3 * m2)), e2 = fma(k3, m1, -(k1 * m3)), e3 = fma(k1, m2, -(k2 * m1));
+    const double hh = fma(h1, h1, fma(h2, h2, h3 * h3));
+    const double ee = fma(e1, e1, fma(e2, e2, e3 * e3));
+    const double ihh = rsq64(hh);
+    const double p = fma(h1, m1, fma(h2, m2, h3 * m3)) * ihh;
+    const double qh = fma(h1, e1, fma(h2, e2, h3 * e3)) * (ihh * rsq64(ee));
+    const double gg = fma(p, d11, qh * d12), hj = fma(p, d12, -(qh * d11));   // rot4 . diff_ampl
+    // contour_integral (diff_mod.f90:357-421), edge by edge as the reference sums: the edge
+    // terms cancel, so their roundings are kept too
+    double sb, cb;
+    sincos_tab64(bvk, tab, &sb, &cb);
+    const double lam = g.lambda, ilam = g.inv_lambda;
+    const double tkx = 2 * kxx, tky = 2 * kyy;
+    double ar = 0.0, ai = 0.0;
+    const auto edge = [&](double xj, double yj, double mj, double nj, double dx, double dy) {
+#pragma clang fp contract(off)
+        const double delta = kxx * xj + kyy * yj;
+        const double delta1 = kyy * mj + kxx;
+        const double delta2 = kxx * nj + kyy;
+        const double omega1 = dx * delta1;
+        const double omega2 = dy * delta2;
+        const double alpha = 1 / (tky * delta1);
+        const double beta = 1 / (tkx * delta2);
+        double s0, c0, s1, c1, s2, c2;
+        sincos_tab64(delta, tab, &s0, &c0);
+        sincos_tab64(delta + omega1, tab, &s1, &c1);
+        sincos_tab64(delta + omega2, tab, &s2, &c2);
+        const double sumim = alpha * (c0 - c1) - beta * (c0 - c2);
+        const double sumre = -(alpha * (s0 - s1)) + beta * (s0 - s2);
+        ar = ar + div_by(cb * sumre - sb * sumim, lam, ilam);
+        ai = ai + div_by(cb * sumim + sb * sumre, lam, ilam);
+    };
+    // triangles: edges inline in the record; polygons: the vertex records.  Two
+    // loops, so each reads one address space (records staged in LDS, vrecs global).
+    const int nv = (int)r[R_NV];
+    if (nv <= 3) {
+        for (int j = 0; j < nv; j++)
+            edge(r[R_EX + j], r[R_EY + j], r[R_EM + j], r[R_EN + j], r[R_EDX + j], r[R_EDY + j]);
+    } else {
+        const double* vr = vrecs + (int64_t)r[R_VOFF] * VREC;
+        for (int j = 0; j < nv; j++) {
+            const double* v = vr + VREC * j;
+            edge(v[V_X], v[V_Y], v[V_M], v[V_N], v[V_DX], v[V_DY]);
+        }
+    }
+    // (g I + h J) . ampl, times the aperture integral
+    const double* A = r + R_AMPL;
+    double o[8];
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+        for (int ri = 0; ri < 2; ri++) {
+            const double a1 = A[4 * q + ri], a2 = A[4 * q + 2 + ri];   // A(1,q), A(2,q)
+            o[4 * q + ri] = fma(gg, a1, hj * a2);
+            o[4 * q + 2 + ri] = fma(gg, a2, -(hj * a1));
+        }
+    if (incoherent) {
+        // diagnostic: sum over beams of |amplC|_F^2 (conditioning of the coherent sum)
+        double n2 = 0.0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) n2 = fma(o[q], o[q], n2);
+        acc[0] = fma(n2, fma(ar, ar, ai * ai), acc[0]);
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const double br = o[2 * q], bim = o[2 * q + 1];
+        acc[2 * q] = fma(br, ar, fma(-bim, ai, acc[2 * q]));
+        acc[2 * q + 1] = fma(br, ai, fma(bim, ar, acc[2 * q + 1]));
+    }
 }
 
 // Difference form of one edge of the rotation-invariant form factor, from the full vertex

@@ -89,6 +89,7 @@ struct DevGrid {
     float inv_krev_f;                       // 1/krev
     double inv_krev;                        // 1/krev (restructured FP64 body)
     double tri_scale;                       // 1 / (2 pi^2 lambda) (its triangle form factor)
+    double inv_lambda;                      // 1 / lambda (plain FP64 body)
     const double* x;                        // [nbins] far-field bin positions
     const double* y;
     const double* z;
@@ -98,13 +99,16 @@ struct DevGrid {
     const double* kr_rev;                   // [nbins] k|x|/(2 pi)
     const double* theta;                    // [ntheta] theta_vals, rad
     const double* phi;                      // [nphi] phi_vals, rad
-    const double* etab;                     // [ETAB_DOUBLES] e^{i pi j / ETAB_N} (restructured FP64 body)
+    const double* etab;                     // [ETAB_GRID] e^{i pi j / ETAB_N}, then its low parts (FP64 bodies)
 };
 
-// Table of the restructured FP64 body's exponentials: cos and sin of pi j / ETAB_N for
-// j = -ETAB_N .. ETAB_N, interleaved, built on the host with the C library.
+// Table of the FP64 bodies' exponentials: cos and sin of pi j / ETAB_N for j = -ETAB_N .. ETAB_N,
+// interleaved and correctly rounded, followed by the low parts (the values less the doubles, same
+// layout) that make each entry a double-double; built on the host in extended precision.
 constexpr int ETAB_N = 512;
 constexpr int ETAB_DOUBLES = 2 * (2 * ETAB_N + 1);
+constexpr int ETAB_LO = ETAB_DOUBLES;       // offset of the low parts
+constexpr int ETAB_GRID = 2 * ETAB_DOUBLES;  // doubles of DevGrid::etab, and the plain body's LDS block
 // The FP64r body's LDS block: the table, then its grid constants (fp64r_constants).
 constexpr int ETAB_KREV = ETAB_DOUBLES, ETAB_K2CLAMP = ETAB_DOUBLES + 1, ETAB_INV_KREV = ETAB_DOUBLES + 2,
               ETAB_TRI_SCALE = ETAB_DOUBLES + 3, ETAB_LAMBDA = ETAB_DOUBLES + 4, ETAB_LDS = ETAB_DOUBLES + 5;
@@ -128,6 +132,7 @@ inline void devgrid_constants(DevGrid& g, double lambda)
     g.inv_krev_f = (float)(1.0 / g.krev);
     g.inv_krev = 1.0 / g.krev;
     g.tri_scale = 1.0 / (2.0 * 9.869604401089358 * lambda);
+    g.inv_lambda = 1.0 / lambda;
 }
 
 }  // namespace abt

@@ -231,7 +231,7 @@ static int grid_build(int64_t nth, const double* th, int64_t nph, const double* 
     int rc = ensure_device();
     if (rc) return rc;
     const int64_t nb = nth * nph;
-    std::vector<double> h((size_t)(5 * nb + 3 * nph + nth + abt::ETAB_DOUBLES));
+    std::vector<double> h((size_t)(5 * nb + 3 * nph + nth + abt::ETAB_GRID));
     double *x = h.data(), *y = x + nb, *z = y + nb, *rad = z + nb, *krr = rad + nb;
     double *cp = krr + nb, *sp = cp + nph, *phv = sp + nph, *thv = phv + nph, *etab = thv + nth;
     abt::fill_exp_table(etab);

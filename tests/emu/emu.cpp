@@ -18,9 +18,10 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
     memset(&g, 0, sizeof(g));
     g.nphi = nphi; g.ntheta = ntheta; g.nbins = nphi * ntheta; g.theta_i = theta_i;
     devgrid_constants(g, lambda);
-    static double etab[ETAB_LDS];
+    static double etab[ETAB_GRID], blk4[ETAB_LDS];   // the grid's table; the FP64r body's LDS block
     fill_exp_table(etab);
-    fp64r_constants(g, etab + ETAB_DOUBLES);
+    memcpy(blk4, etab, sizeof(double) * ETAB_DOUBLES);
+    fp64r_constants(g, blk4 + ETAB_DOUBLES);
     g.etab = etab;
     const int nbins = g.nbins;
 #pragma omp parallel for schedule(dynamic, 64)
@@ -33,9 +34,9 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
         float accf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int64_t b = 0; b < nb; b++) {
             if (prec == 0)
-                beam_bin_fp64(recs + b * REC, vrecs, g, bi, cphi[iphi], sphi[iphi], fast_diff, acc);
+                beam_bin_fp64(recs + b * REC, vrecs, g, bi, etab, fast_diff, acc);
             else if (prec == 4)     // the restructured FP64 body
-                beam_bin_fp64r(recs + b * REC, vrecs, g, bi, g.etab, fast_diff, acc);
+                beam_bin_fp64r(recs + b * REC, vrecs, g, bi, blk4, fast_diff, acc);
             else if (prec == 3) {   // the FP64 model (adjbody.h)
                 AdjBeam B;
                 adj_beam_from_record(recs + b * REC, B);
@@ -50,6 +51,7 @@ extern "C" void emu_far_field(int prec, int nphi, int ntheta, int theta_i, doubl
                     }
             }
         }
+        if (prec == 0) apply_temp_rot1(acc, cphi[iphi], sphi[iphi]);
         if (prec == 1) mixed_finish(acc, cphi[iphi], sphi[iphi]);
         if (prec == 4) fp64r_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
         if (prec == 3) model_finish(acc, cphi[iphi], sphi[iphi], g.krev * rad);
