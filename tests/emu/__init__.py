@@ -65,3 +65,22 @@ def adjoint(recs, bins, lam_p, lam, mixed=False):
     f(P(recs), ctypes.c_int64(nb), P(bins), P(lam_p), ctypes.c_int64(bins.shape[0]), ctypes.c_double(lam),
       P(gv), P(ga))
     return gv.reshape(nb, 3, 3), ga.reshape(nb, 8)
+
+
+def sincos_tab(x):
+    """sincos_tab64 of the plain FP64 body on the CPU: (sin x, cos x)."""
+    x = np.ascontiguousarray(x, np.float64)
+    s, c = np.empty_like(x), np.empty_like(x)
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib().emu_sincos_tab(ctypes.c_int64(x.size), P(x), P(s), P(c))
+    return s, c
+
+
+def div_by(a, b):
+    """div_by(a, b, 1/b) of the plain FP64 body on the CPU."""
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    q = np.empty_like(a)
+    P = lambda v: v.ctypes.data_as(ctypes.c_void_p)
+    lib().emu_div_by(ctypes.c_int64(a.size), P(a), P(b), P(q))
+    return q

@@ -115,3 +115,23 @@ extern "C" void emu_adjoint_mixed(const double* recs, int64_t nb, const double* 
         adj_finish(B, r + R_V20, r[R_SIGMA], acc, gv + 9 * b, ga + 8 * b);
     }
 }
+
+// The plain FP64 body's helpers on the CPU: the table sine and cosine (sincos_tab64 over the
+// grid's table) and the quotient by a correctly rounded reciprocal (div_by).
+extern "C" void emu_sincos_tab(int64_t n, const double* x, double* s, double* c)
+{
+    using namespace abt;
+    static double tab[ETAB_GRID];
+    static bool filled = false;
+    if (!filled) {
+        fill_exp_table(tab);
+        filled = true;
+    }
+    for (int64_t i = 0; i < n; i++) sincos_tab64(x[i], tab, s + i, c + i);
+}
+
+extern "C" void emu_div_by(int64_t n, const double* a, const double* b, double* q)
+{
+    using namespace abt;
+    for (int64_t i = 0; i < n; i++) q[i] = div_by(a[i], b[i], 1.0 / b[i]);
+}
