@@ -2,10 +2,12 @@
 
 Tolerances
   FP64 kernel: per-bin amplitude error <= 1e-9 (floored at 1e-3 of the largest
-  bin).  The reference's split contour formula (diff_mod.f90:408-420) cancels
-  terms of size 1/kxx, 1/kyy; bins whose |q| a is below 1e-3 for some beam (the
-  exact forward direction of a beam) carry the formula's own ~1e-16/(|q| a)^2
-  rounding and are checked at 1e-6 instead.
+  bin), on every bin.  The reference's split contour formula (diff_mod.f90:408-420)
+  cancels terms of size 1/kxx, 1/kyy, most strongly where |q| a < 1e-3 for some beam
+  (the exact forward direction of a beam, ~1e-16/(|q| a)^2 of rounding); the plain
+  body keeps the reference's operations for those terms and for every phase, so it
+  reproduces that rounding: measured at most 1.2e-10 off those directions (the full
+  reference CPU case) and 1.6e-13 on them.
   Mixed kernel: see check_mixed (amplitude <= 6e-6, normalised Mueller
   elements |dS_ij| / S11 <= 1e-5 on bright, non-forward bins).  These are the
   mixed kernel's own criteria; they are looser than the accuracy gate
@@ -64,8 +66,8 @@ def _forward_mask(th, ph, beams, tol_qa=1e-3):
 def _check_fp64(got, ref, fwd):
     e = ampl_rel_err(got, ref)
     assert e[~fwd].max() <= 1e-9, summary(e[~fwd])
-    if fwd.any():
-        assert e[fwd].max() <= 1e-6, summary(e[fwd])
+    if fwd.any():   # the exact forward directions, reported apart
+        assert e[fwd].max() <= 1e-9, summary(e[fwd])
 
 
 def test_fp64_diff_main_matches_oracle_column(orc):
