@@ -37,16 +37,31 @@ def test_emu_fp64_body_matches_oracle(orc):
 
 def test_table_sincos_within_half_ulp():
     """sincos_tab64, the plain body's sine and cosine, against x87 extended precision over the
-    ranges the body sees (edge phases to ~1e4 rad, the far-field phase k|x - c| ~ 1.2e7 rad):
-    within 6e-17 absolute, i.e. about half an ulp.  A table without its low parts errs by up to
-    1.7e-16, a one-constant reduction by ~5e-10 at 1.2e7 rad."""
+    ranges the body sees (edge phases to ~1e4 rad, the far-field phase k|x - c| ~ 1.2e7 rad at
+    0.532 um) and beyond, where the table index passes 2^31 and wraps in the low bits of the
+    shifted sum (4e10 rad): within 6e-17 absolute, i.e. about half an ulp.  A table without its
+    low parts errs by up to 1.7e-16, a one-constant reduction by ~5e-10 at 1.2e7 rad."""
     rng = np.random.default_rng(3)
-    for span in (10.0, 3e3, 1.3e7):
+    for span in (10.0, 3e3, 1.3e7, 4e10):
         x = rng.uniform(-span, span, 200_000)
         s, c = emu.sincos_tab(x)
         xl = x.astype(np.longdouble)
         assert float(np.abs(s - np.sin(xl)).max()) <= 6e-17, span
         assert float(np.abs(c - np.cos(xl)).max()) <= 6e-17, span
+
+
+@pytest.mark.parametrize("lam", [0.2, 1.5])
+def test_emu_fp64_body_other_wavelengths(orc, lam):
+    """The plain body against the oracle at a short wavelength (k |x - c| ~ 3e7 rad, table indices
+    past 2^31) and a long one: within 1e-9 of the floored amplitude on every bin (measured 1.3e-12
+    and 1.6e-11)."""
+    th, ph = _coarse()
+    b, _ = _column_case()
+    ref, _ = orc.diff_main(th, ph, lam, b, BeamSet.empty(), nthreads=8)
+    recs, vr = lib.prepare_beams_host(b, lam)
+    got = emu.far_field(0, th, ph, lam, recs, b.n, vr)
+    err = ampl_rel_err(got, ref)
+    assert err.max() <= 1e-9, summary(err)
 
 
 def test_markstein_quotient_equals_division():
