@@ -448,16 +448,26 @@ __host__ __device__ __forceinline__ double rsq64_1(double x)
     return fma(0.5 * y, fma(-x * y, y, 1.0), y);
 }
 
-// The table of e^{i pi j / ETAB_N} (DevGrid::etab, ETAB_GRID doubles), built by the host: the
-// angle and its cosine and sine in x87 extended precision (64-bit significands), rounded once to
-// double, and the remainders (the low parts, at ETAB_LO).
+// The table of e^{i pi j / ETAB_N} (DevGrid::etab, ETAB_GRID doubles), built by the host in x87
+// extended precision (64-bit significands): each angle reduced exactly to at most pi/4 (quarter
+// turns, and the reflection a -> pi/2 - a within a quarter), so the extended-precision angle is
+// within ~1e-19 of the exact one; cosine and sine rounded once to double, and the remainders (the
+// low parts, at ETAB_LO).
 inline void fill_exp_table(double* tab)
 {
     const long double pi_l = 3.14159265358979323846264338327950288L;
-    for (int j = -ETAB_N; j <= ETAB_N; j++) {
-        const long double a = pi_l * (long double)j / (long double)ETAB_N;
-        const long double c = cosl(a), s = sinl(a);
-        const int i = 2 * (j + ETAB_N);
+    const int quarter = ETAB_N / 2;
+    for (int j = 0; j < 2 * ETAB_N; j++) {
+        const int q = j / quarter, m = j % quarter;
+        const bool refl = 2 * m > quarter;
+        const long double a = pi_l * (long double)(refl ? quarter - m : m) / (long double)ETAB_N;
+        long double c = refl ? sinl(a) : cosl(a), s = refl ? cosl(a) : sinl(a);
+        for (int k = 0; k < q; k++) {   // rotate by a quarter turn: (c, s) -> (-s, c)
+            const long double t = c;
+            c = -s;
+            s = t;
+        }
+        const int i = 2 * j;
         tab[i] = (double)c;
         tab[i + 1] = (double)s;
         tab[ETAB_LO + i] = (double)(c - (long double)tab[i]);
@@ -485,7 +495,7 @@ __host__ __device__ __forceinline__ void expi_pi64(double t, const double* __res
     const double u = fma(-2.0, m, t);                       // [-1, 1]
     const double jf = rint((double)ETAB_N * u);
     const double r = fma(-1.0 / (double)ETAB_N, jf, u);     // |r| <= 1 / (2 ETAB_N)
-    const int j = (int)jf + ETAB_N;
+    const int j = (int)jf & (2 * ETAB_N - 1);
     const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
     const double y = PI64 * r;
     const double z = y * y;
@@ -498,17 +508,22 @@ __host__ __device__ __forceinline__ void expi_pi64(double t, const double* __res
 // e^{2 pi i t} for an FP64 t (revolutions, |t| < 2^20), from the same table: t = J / (2 ETAB_N) + r
 // with J = rint(2 ETAB_N t) and |r| <= 1 / (4 ETAB_N) (exact: t and J / (2 ETAB_N) are multiples of
 // the ulp of t), e^{2 pi i t} = e^{2 pi i J / (2 ETAB_N)} e^{2 pi i r}; the table holds the angle
-// 2 pi J / (2 ETAB_N) modulo 2 pi at index (J + ETAB_N) mod 2 ETAB_N.
+// 2 pi J / (2 ETAB_N) modulo 2 pi at index J mod 2 ETAB_N.  e^{2 pi i r} from its Taylor
+// polynomials in r itself, 2 pi folded into the coefficients (truncation below 2e-18).
+constexpr double EXP_S1 = 6.283185307179586;     // 2 pi
+constexpr double EXP_S3 = -41.34170224039976;    // -(2 pi)^3 / 6
+constexpr double EXP_S5 = 81.60524927607506;     // (2 pi)^5 / 120
+constexpr double EXP_C2 = -19.739208802178716;   // -(2 pi)^2 / 2
+constexpr double EXP_C4 = 64.9393940226683;      // (2 pi)^4 / 24
 __host__ __device__ __forceinline__ void expi_2pi64(double t, const double* __restrict__ tab, double* c, double* s)
 {
     const double jf = rint((double)(2 * ETAB_N) * t);
     const double r = fma(-1.0 / (double)(2 * ETAB_N), jf, t);     // |r| <= 1 / (4 ETAB_N)
-    const int j = ((int)jf + ETAB_N) & (2 * ETAB_N - 1);
+    const int j = (int)jf & (2 * ETAB_N - 1);
     const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
-    const double y = (2.0 * PI64) * r;
-    const double z = y * y;
-    const double sr = fma(y * z, fma(z, 1.0 / 120.0, -1.0 / 6.0), y);
-    const double cr = fma(z, fma(z, 1.0 / 24.0, -0.5), 1.0);
+    const double w = r * r;
+    const double sr = r * fma(w, fma(w, EXP_S5, EXP_S3), EXP_S1);
+    const double cr = fma(w, fma(w, EXP_C4, EXP_C2), 1.0);
     *c = fma(c0, cr, -s0 * sr);
     *s = fma(s0, cr, c0 * sr);
 }
@@ -530,7 +545,7 @@ __host__ __device__ __forceinline__ void sincos_tab64(double x, const double* __
     const double n = t - SINCOS_SHIFT;
     double r = fma(-n, PI_OVER_N_HI, x);
     r = fma(-n, PI_OVER_N_LO, r);
-    const int j = ((int)(uint32_t)__builtin_bit_cast(uint64_t, t) + ETAB_N) & (2 * ETAB_N - 1);
+    const int j = (int)(uint32_t)__builtin_bit_cast(uint64_t, t) & (2 * ETAB_N - 1);
     const double c0 = tab[2 * j], s0 = tab[2 * j + 1];
     const double cl = tab[ETAB_LO + 2 * j], sl = tab[ETAB_LO + 2 * j + 1];
     const double z = r * r;

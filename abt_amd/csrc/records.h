@@ -102,11 +102,12 @@ struct DevGrid {
     const double* etab;                     // [ETAB_GRID] e^{i pi j / ETAB_N}, then its low parts (FP64 bodies)
 };
 
-// Table of the FP64 bodies' exponentials: cos and sin of pi j / ETAB_N for j = -ETAB_N .. ETAB_N,
-// interleaved and correctly rounded, followed by the low parts (the values less the doubles, same
-// layout) that make each entry a double-double; built on the host in extended precision.
+// Table of the FP64 bodies' exponentials: cos and sin of pi j / ETAB_N for j = 0 .. 2 ETAB_N - 1
+// (one turn, so an angle's index is its multiple of pi / ETAB_N modulo 2 ETAB_N), interleaved and
+// correctly rounded, followed by the low parts (the values less the doubles, same layout) that
+// make each entry a double-double; built on the host in extended precision.
 constexpr int ETAB_N = 512;
-constexpr int ETAB_DOUBLES = 2 * (2 * ETAB_N + 1);
+constexpr int ETAB_DOUBLES = 2 * (2 * ETAB_N);
 constexpr int ETAB_LO = ETAB_DOUBLES;       // offset of the low parts
 constexpr int ETAB_GRID = 2 * ETAB_DOUBLES;  // doubles of DevGrid::etab, and the plain body's LDS block
 // The FP64r body's LDS block: the table, then its grid constants (fp64r_constants).
