@@ -52,9 +52,10 @@ def test_table_sincos_within_half_ulp():
 
 @pytest.mark.parametrize("lam", [0.2, 1.5])
 def test_emu_fp64_body_other_wavelengths(orc, lam):
-    """The plain body against the oracle at a short wavelength (k |x - c| ~ 3e7 rad, table indices
-    past 2^31) and a long one: within 1e-9 of the floored amplitude on every bin (measured 1.3e-12
-    and 1.6e-11)."""
+    """The FP64 bodies against the oracle at a short wavelength (k |x - c| ~ 3e7 rad, table
+    indices past 2^31) and a long one.  Plain: within 1e-9 of the floored amplitude on every bin
+    (measured 1.4e-12 and 1.6e-11).  Restructured: within 1e-7 off the exact forward directions
+    (measured 2.0e-8 and 7.2e-9)."""
     th, ph = _coarse()
     b, _ = _column_case()
     ref, _ = orc.diff_main(th, ph, lam, b, BeamSet.empty(), nthreads=8)
@@ -62,6 +63,10 @@ def test_emu_fp64_body_other_wavelengths(orc, lam):
     got = emu.far_field(0, th, ph, lam, recs, b.n, vr)
     err = ampl_rel_err(got, ref)
     assert err.max() <= 1e-9, summary(err)
+    got = emu.far_field(4, th, ph, lam, recs, b.n, vr)
+    err = ampl_rel_err(got, ref)
+    fwd = _forward_mask(th, ph, [b], lam=lam)
+    assert err[~fwd].max() <= 1e-7, summary(err[~fwd])
 
 
 def test_markstein_quotient_equals_division():
