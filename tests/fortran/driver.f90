@@ -2,7 +2,8 @@
 ! tests/fortran/__init__.py, fills the reference's own data types and calls
 ! diff_main, diffraction and make_far_field_bins exactly as PBT's drivers do
 ! (pbt_seq_mod.f90:212-218, beam_loop_mod.f90:44, :1865), then writes the
-! results.  Usage: abt_driver <input.bin> <output.bin> <log.txt>
+! results.  Usage: abt_driver <input.bin> <output.bin> <log.txt>; ABT_DRIVER_REPEAT=n calls
+! diff_main n times (the reference drivers' once per orientation), each writing its timing lines.
 program abt_driver
     use types_mod
     use diff_mod
@@ -17,6 +18,8 @@ program abt_driver
     real(8), dimension(:,:), allocatable :: x, y, z
     integer(8), dimension(:), allocatable :: fnv, fvi
     real(8), dimension(:), allocatable :: buf
+    character(len=16) :: rep_s
+    integer :: rep, ios, r
 
     call get_command_argument(1, fin)
     call get_command_argument(2, fout)
@@ -53,8 +56,14 @@ program abt_driver
     call read_tree(ext, ne)
     close(10)
 
+    rep = 1
+    call get_environment_variable('ABT_DRIVER_REPEAT', rep_s, status=ios)
+    if (ios == 0) read(rep_s, *, iostat=ios) rep
+    if (ios /= 0 .or. rep < 1) rep = 1
     open(101, file=trim(flog))
-    call diff_main(beams, nb, ampl_far_beam, ext, ampl_far_ext_diff, job_params, geometry)
+    do r = 1, rep
+        call diff_main(beams, nb, ampl_far_beam, ext, ampl_far_ext_diff, job_params, geometry)
+    end do
     call make_far_field_bins(x, y, z, job_params%theta_vals, job_params%phi_vals)
     allocate(amplc(nph, nth, 2, 2))
     if (nb > 0) then
