@@ -639,12 +639,13 @@ int abt_far_field_incoherent_dev(const abt_dev_grid* g, const double* dev_recs,
 int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff, const abt_beams* beams,
                   const abt_beams* ext, int precision, double* ampl_far_beam, double* ampl_far_ext_diff)
 {
-    if (!grid || !ampl_far_beam || !ampl_far_ext_diff) return fail(ABT_ERR_ARG, "abt_diff_main: null argument");
+    const bool want_b = ampl_far_beam != nullptr, want_e = ampl_far_ext_diff != nullptr;
+    if (!grid || (!want_b && !want_e)) return fail(ABT_ERR_ARG, "abt_diff_main: null argument");
     if (precision != ABT_FP64 && precision != ABT_MIXED && precision != ABT_FP64_FAST)
         return fail(ABT_ERR_ARG, "unknown precision %d", precision);
-    int rc = check_beams_host(beams, "abt_diff_main(beams)");
+    int rc = want_b ? check_beams_host(beams, "abt_diff_main(beams)") : 0;
     if (rc) return rc;
-    rc = check_beams_host(ext, "abt_diff_main(ext)");
+    rc = want_e ? check_beams_host(ext, "abt_diff_main(ext)") : 0;
     if (rc) return rc;
     if (!grid->theta || !grid->phi) return fail(ABT_ERR_ARG, "abt_diff_main: null angles");
     abt_dev_grid* g = nullptr;
@@ -659,21 +660,22 @@ int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff, const a
     if (!rc) {
         double* d0 = (double*)ts.bufs[0].p;
         double* d1 = d0 + 8 * g->nbins;
-        rc = run_host_tree(g->d, beams, 0.0, precision, is_fast_diff, d0, ts, 1);
-        if (!rc) {
+        if (want_b) rc = run_host_tree(g->d, beams, 0.0, precision, is_fast_diff, d0, ts, 1);
+        if (!rc && want_b && want_e) {
             // the ext tree's staging must not overwrite the beam tree's still in flight
             hipError_t e = hipStreamSynchronize(ts.st);
             if (e != hipSuccess) rc = fail(ABT_ERR_HIP, "stream sync: %s", hipGetErrorString(e));
         }
         // ext-diff beams: fov = pi/2 (diff_mod.f90:797)
-        if (!rc) {
+        if (!rc && want_e) {
             abt_beams e2 = *ext;
             e2.fov = nullptr;
             rc = run_host_tree(g->d, &e2, abt::PI_R4 / 2.0, precision, is_fast_diff, d1, ts, 1);
         }
         if (!rc) {
-            hipError_t e = hipMemcpyAsync(ampl_far_beam, d0, ob, hipMemcpyDeviceToHost, ts.st);
-            if (e == hipSuccess) e = hipMemcpyAsync(ampl_far_ext_diff, d1, ob, hipMemcpyDeviceToHost, ts.st);
+            hipError_t e = hipSuccess;
+            if (want_b) e = hipMemcpyAsync(ampl_far_beam, d0, ob, hipMemcpyDeviceToHost, ts.st);
+            if (e == hipSuccess && want_e) e = hipMemcpyAsync(ampl_far_ext_diff, d1, ob, hipMemcpyDeviceToHost, ts.st);
             if (e == hipSuccess) e = hipStreamSynchronize(ts.st);
             if (e != hipSuccess) rc = fail(ABT_ERR_HIP, "copy-back: %s", hipGetErrorString(e));
         }

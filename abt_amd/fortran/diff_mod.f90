@@ -221,7 +221,7 @@ module diff_mod
         real(8), dimension(:), allocatable, target :: th, ph
         real(8), dimension(:), allocatable, target :: ba, bk, bp, bf, bv, ea, ek, ep, ef, ev
         integer(c_int64_t), dimension(:), allocatable, target :: bo, eo
-        complex(8), dimension(:,:,:,:), allocatable, target :: ab, ae, scratch
+        complex(8), dimension(:,:,:,:), allocatable, target :: ab, ae
         type(abt_grid_c) :: grid
         type(abt_beams_c) :: beams, ext, none
         integer(8) :: nph, nth, ne
@@ -240,7 +240,7 @@ module diff_mod
         if (ne > 0) then
             call pack_beams(ext_diff_outbeam_tree, ne, geometry, ea, ek, ep, ef, eo, ev)
         end if
-        allocate(ab(nph, nth, 2, 2), ae(nph, nth, 2, 2), scratch(nph, nth, 2, 2))
+        allocate(ab(nph, nth, 2, 2), ae(nph, nth, 2, 2))
         grid = abt_grid_c(nth, nph, c_loc(th), c_loc(ph))
         beams = abt_beams_c(beam_outbeam_tree_counter, c_loc(ba), c_loc(bk), c_loc(bp), c_loc(bf), &
                             c_loc(bo), c_loc(bv))
@@ -258,8 +258,9 @@ module diff_mod
 
         if (job_params%debug >= 1) write(101,*)'start diff beam loop...'
         if (job_params%timing) start1 = omp_get_wtime()
+        ! one tree per call, so that each gets the reference's own timing line
         call check(abt_diff_main_c(grid, job_params%la, fast, beams, none, prec, c_loc(ab), &
-                                   c_loc(scratch)), 'diff_main (beam tree)')
+                                   c_null_ptr), 'diff_main (beam tree)')
         if (job_params%debug >= 1) then
             write(101,*)'end diff beam loop...'
             if (job_params%timing) then
@@ -270,7 +271,7 @@ module diff_mod
             write(101,*)'start ext diff loop...'
         end if
         ! ext-diff beams take fov = pi/2 inside the library (reference diff_mod.f90:797)
-        call check(abt_diff_main_c(grid, job_params%la, fast, none, ext, prec, c_loc(scratch), &
+        call check(abt_diff_main_c(grid, job_params%la, fast, none, ext, prec, c_null_ptr, &
                                    c_loc(ae)), 'diff_main (ext diff tree)')
         if (job_params%debug >= 1) then
             write(101,*)'end ext diff loop...'

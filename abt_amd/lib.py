@@ -169,18 +169,22 @@ def make_far_field_bins(theta, phi):
 
 
 def diff_main(theta, phi, lam, beams: BeamSet, ext: BeamSet, precision="fp64", fast_diff=False,
-              raw=False):
-    """diff_main (diff_mod.f90:674-825): (ampl_far_beam, ampl_far_ext_diff)."""
+              raw=False, trees=("beam", "ext")):
+    """diff_main (diff_mod.f90:674-825): (ampl_far_beam, ampl_far_ext_diff).  trees: the trees to
+    compute; an output left out is None (the C-ABI's NULL output)."""
     theta, phi = _f64(theta), _f64(phi)
     nb = len(theta) * len(phi)
-    ob, oe = np.empty(8 * nb), np.empty(8 * nb)
+    ob = np.empty(8 * nb) if "beam" in trees else None
+    oe = np.empty(8 * nb) if "ext" in trees else None
     g = grid_struct(theta, phi)
     _check(load().abt_diff_main(ctypes.byref(g), lam, int(fast_diff),
                                 ctypes.byref(beams_struct(beams)), ctypes.byref(beams_struct(ext)),
-                                PRECISIONS[precision], _ptr(ob), _ptr(oe)))
+                                PRECISIONS[precision], _ptr(ob) if ob is not None else None,
+                                _ptr(oe) if oe is not None else None))
     if raw:
         return ob, oe
-    return raw_to_complex(ob, len(phi), len(theta)), raw_to_complex(oe, len(phi), len(theta))
+    conv = lambda o: None if o is None else raw_to_complex(o, len(phi), len(theta))
+    return conv(ob), conv(oe)
 
 
 def diffraction(theta, phi, x, y, z, lam, beams: BeamSet, b, precision="fp64", fast_diff=False):

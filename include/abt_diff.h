@@ -99,7 +99,9 @@ int abt_make_far_field_bins(int64_t n_theta, const double* theta, int64_t n_phi,
 /* diff_main (diff_mod.f90:674-825).  Sums every beam of `beams` into
  * ampl_far_beam and every beam of `ext` (fov forced to pi/2, diff_mod.f90:797)
  * into ampl_far_ext_diff, both overwritten.  Beams are summed in tree order
- * for every bin (deterministic).  is_fast_diff: FOV culling (diff_mod.f90:593-603). */
+ * for every bin (deterministic).  is_fast_diff: FOV culling (diff_mod.f90:593-603).
+ * One of the two outputs may be NULL: that tree is then neither computed nor copied
+ * back, and its beam list is not read (callers that time the two trees apart). */
 int abt_diff_main(const abt_grid* grid, double lambda, int is_fast_diff,
                   const abt_beams* beams, const abt_beams* ext, int precision,
                   double* ampl_far_beam, double* ampl_far_ext_diff);

@@ -67,6 +67,23 @@ def test_grid_cache_follows_inputs(orc):
         _check_fp64(got, ref, _forward_mask(th, phk, [b.subset([0])]))
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp64_fast", "mixed"])
+def test_diff_main_one_tree_per_call(precision):
+    """abt_diff_main with one output NULL computes only the other tree (the Fortran drop-in
+    calls it once per tree, to write the reference's per-tree timing lines): each output equals
+    the two-tree call's bit for bit."""
+    th, ph = grid((0, 5, 180), (0, 10, 350))
+    b = random_triangles(20, seed=41, box=10.0)
+    e = random_triangles(6, seed=42, box=10.0)
+    gb, ge = lib.diff_main(th, ph, LAM, b, e, precision=precision)
+    ob, none = lib.diff_main(th, ph, LAM, b, e, precision=precision, trees=("beam",))
+    assert none is None
+    np.testing.assert_array_equal(ob, gb)
+    none, oe = lib.diff_main(th, ph, LAM, b, e, precision=precision, trees=("ext",))
+    assert none is None
+    np.testing.assert_array_equal(oe, ge)
+
+
 def test_diff_main_on_beams_loaded_from_pbt_export(orc):
     """The outgoing beams read from a file in the layout of PBT's -export_beam
     (tests/golden/beam_export.json) through diff_main, with and without -fast_diff (which

@@ -104,6 +104,10 @@ def test_bad_arguments_raise():
     bs = random_triangles(2, seed=1)
     with pytest.raises(lib.AbtError, match="wavelength"):
         lib.prepare_beams_host(bs, -1.0)
+    # diff_main needs at least one of its two outputs (rejected before any device work)
+    th = np.radians([0.0, 1.0, 2.0])
+    with pytest.raises(lib.AbtError, match="null argument"):
+        lib.diff_main(th, th, LAM, bs, BeamSet.empty(), trees=())
 
 
 def test_beam_set_files_round_trip(tmp_path):
