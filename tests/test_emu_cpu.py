@@ -69,6 +69,20 @@ def test_emu_fp64_body_other_wavelengths(orc, lam):
     assert err[~fwd].max() <= 1e-7, summary(err[~fwd])
 
 
+def test_emu_exact_forward_with_clamped_k(orc):
+    """The plain body on the reference's clamped exact-forward bins (on_axis_triangles): within
+    1e-9 of the oracle (measured 3e-16), where the exact form factor differs by ~1e-5."""
+    from tests.test_gpu_parity import on_axis_triangles
+    th = np.radians([0.0, 0.5, 1.0, 2.0])
+    ph = np.radians(np.arange(0.0, 360.0, 30.0))
+    bs = on_axis_triangles()
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=4)
+    recs, vr = lib.prepare_beams_host(bs, LAM)
+    err = ampl_rel_err(emu.far_field(0, th, ph, LAM, recs, bs.n, vr), ref)
+    assert err.max() <= 1e-9, summary(err)
+    assert ampl_rel_err(emu.far_field(4, th, ph, LAM, recs, bs.n, vr), ref)[0].max() > 1e-7
+
+
 def test_markstein_quotient_equals_division():
     """div_by, the plain body's quotient by a shared divisor (q = a y, q + (a - b q) y with
     y = 1/b correctly rounded), is bit-equal to IEEE division: on 10^6 pairs over 26 decades,

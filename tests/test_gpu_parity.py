@@ -117,6 +117,39 @@ def test_fp64_single_diffraction_and_fast_diff(orc):
             _check_fp64(got, ref, _forward_mask(th, ph, [b.subset([i])]))
 
 
+def on_axis_triangles(n=6, seed=2):
+    """Triangles centred on the z axis (at random heights and rotations), propagating along -z:
+    seen from the theta = 0 bins, their forward direction is exact, so the reference clamps
+    kxx and kyy to 1e-6 (diff_mod.f90:364-365) and its edge terms carry reciprocals ~1e12."""
+    rng = np.random.default_rng(seed)
+    sets = []
+    for _ in range(n):
+        s, ang = rng.uniform(2, 6), rng.uniform(0, 2 * np.pi)
+        v = np.array([[0, s / np.sqrt(3), 0], [-s / 2, -s / (2 * np.sqrt(3)), 0],
+                      [s / 2, -s / (2 * np.sqrt(3)), 0]])
+        c, sn = np.cos(ang), np.sin(ang)
+        v = v @ np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]]).T
+        v[:, 2] += rng.uniform(-5, 5)
+        sets.append(BeamSet.from_triangles(rng.normal(size=(1, 8)), [(1.0, 0, 0)], [(0, 0, -1.0)], [1.0],
+                                           v[None]))
+    return concat(sets)
+
+
+def test_fp64_exact_forward_with_clamped_k(orc):
+    """The exact forward bins of on_axis_triangles, where the reference's value is set by its
+    own rounding (the exact form factor differs from it by ~1e-5 there): the plain FP64 kernel
+    reproduces it to 1e-9, like every other bin, because it keeps the reference's operations."""
+    th = np.radians([0.0, 0.5, 1.0, 2.0])
+    ph = np.radians(np.arange(0.0, 360.0, 30.0))
+    bs = on_axis_triangles()
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=4)
+    got, _ = lib.diff_main(th, ph, LAM, bs, BeamSet.empty(), precision="fp64")
+    err = ampl_rel_err(got, ref)
+    assert err.max() <= 1e-9, summary(err)
+    exact, _ = lib.diff_main(th, ph, LAM, bs, BeamSet.empty(), precision="fp64_fast")
+    assert ampl_rel_err(exact, ref)[0].max() > 1e-7     # the clamped regime is reached
+
+
 @pytest.mark.parametrize("lam", [0.2, 1.5])
 def test_fp64_bodies_other_wavelengths(orc, lam):
     """Both FP64 bodies away from 0.532 um: at 0.2 um the far-field phase k |x - c| ~ 3e7 rad puts
