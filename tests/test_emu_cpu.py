@@ -83,6 +83,18 @@ def test_emu_exact_forward_with_clamped_k(orc):
     assert ampl_rel_err(emu.far_field(4, th, ph, LAM, recs, bs.n, vr), ref)[0].max() > 1e-7
 
 
+def test_emu_inside_the_ky_clamp(orc):
+    """The plain body on bins inside the reference's k_y clamp (ky_clamp_case): within 1e-9 of
+    the oracle (measured 1.7e-11)."""
+    from tests.test_gpu_parity import ky_clamp_case
+    th, ph, bs, inside = ky_clamp_case()
+    assert inside > 100
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=4)
+    recs, vr = lib.prepare_beams_host(bs, LAM)
+    err = ampl_rel_err(emu.far_field(0, th, ph, LAM, recs, bs.n, vr), ref)
+    assert err.max() <= 1e-9, summary(err)
+
+
 def test_markstein_quotient_equals_division():
     """div_by, the plain body's quotient by a shared divisor (q = a y, q + (a - b q) y with
     y = 1/b correctly rounded), is bit-equal to IEEE division: on 10^6 pairs over 26 decades,

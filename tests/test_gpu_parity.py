@@ -135,6 +135,47 @@ def on_axis_triangles(n=6, seed=2):
     return concat(sets)
 
 
+def ky_clamp_case(seed=3):
+    """Triangles in the xy plane, propagating in the x-z plane, so the aperture frame's y axis is
+    the global +-y; bins on a 0.025 deg grid around theta 90, phi 90 and 270.  Returns the grid,
+    the beams and how many beam.bins fall inside the reference's |k_y| > 0.999999 clamp
+    (diff_mod.f90:151)."""
+    rng = np.random.default_rng(seed)
+    th = np.radians(np.arange(89.9, 90.1001, 0.025))
+    ph = np.radians(np.r_[np.arange(89.9, 90.1001, 0.025), np.arange(269.9, 270.1001, 0.025)])
+    sets = []
+    for _ in range(6):
+        s, ang = rng.uniform(2, 6), rng.uniform(0, 2 * np.pi)
+        v = np.array([[0, s / np.sqrt(3), 0], [-s / 2, -s / (2 * np.sqrt(3)), 0],
+                      [s / 2, -s / (2 * np.sqrt(3)), 0]])
+        c, sn = np.cos(ang), np.sin(ang)
+        v = v @ np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]]).T + rng.uniform(-3, 3, 3) * np.array([1, 0, 1])
+        a = rng.uniform(0.2, 1.0)
+        sets.append(BeamSet.from_triangles(rng.normal(size=(1, 8)), [(0, 1.0, 0)],
+                                           [(np.sin(a), 0.0, -np.cos(a))], [1.0], v[None]))
+    bs = concat(sets)
+    recs, _ = lib.prepare_beams_host(bs, LAM)
+    x, y, z = O.make_far_field_bins(th, ph)
+    X = np.stack([x.ravel(), y.ravel(), z.ravel()], 1)
+    inside = 0
+    for b in range(bs.n):
+        d = X - recs[b, 0:3]
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        inside += int((np.abs(d @ recs[b, 3:12].reshape(3, 3)[1]) > 0.9999989867210388).sum())
+    return th, ph, bs, inside
+
+
+def test_fp64_inside_the_ky_clamp(orc):
+    """Bins inside the reference's k_y clamp (ky_clamp_case: ~45 % of the beam.bins): the plain
+    FP64 kernel within 1e-9 per bin of the oracle."""
+    th, ph, bs, inside = ky_clamp_case()
+    assert inside > 100
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=4)
+    got, _ = lib.diff_main(th, ph, LAM, bs, BeamSet.empty(), precision="fp64")
+    err = ampl_rel_err(got, ref)
+    assert err.max() <= 1e-9, summary(err)
+
+
 def test_fp64_exact_forward_with_clamped_k(orc):
     """The exact forward bins of on_axis_triangles, where the reference's value is set by its
     own rounding (the exact form factor differs from it by ~1e-5 there): the plain FP64 kernel
