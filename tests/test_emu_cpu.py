@@ -95,6 +95,18 @@ def test_emu_inside_the_ky_clamp(orc):
     assert err.max() <= 1e-9, summary(err)
 
 
+def test_emu_clamped_edge_slopes(orc):
+    """The plain body with edges parallel to the aperture frame's axes (axis_edge_triangles):
+    within 1e-9 of the oracle (measured 1.4e-11)."""
+    from tests.test_gpu_parity import axis_edge_triangles
+    th, ph = _coarse()
+    bs = axis_edge_triangles()
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=4)
+    recs, vr = lib.prepare_beams_host(bs, LAM)
+    err = ampl_rel_err(emu.far_field(0, th, ph, LAM, recs, bs.n, vr), ref)
+    assert err.max() <= 1e-9, summary(err)
+
+
 def test_markstein_quotient_equals_division():
     """div_by, the plain body's quotient by a shared divisor (q = a y, q + (a - b q) y with
     y = 1/b correctly rounded), is bit-equal to IEEE division: on 10^6 pairs over 26 decades,

@@ -176,6 +176,35 @@ def test_fp64_inside_the_ky_clamp(orc):
     assert err.max() <= 1e-9, summary(err)
 
 
+def axis_edge_triangles(seed=4):
+    """Right triangles in the xy plane with legs along x and y, propagating in the x-z plane: in
+    the aperture frame one edge is horizontal and one vertical, so their slopes take the
+    reference's clamps (m or n = 1e6, diff_mod.f90:379-382)."""
+    rng = np.random.default_rng(seed)
+    sets = []
+    for _ in range(6):
+        a_, b_ = rng.uniform(2, 6, 2)
+        v = (np.array([[0, 0, 0], [a_, 0, 0], [0, b_, 0.0]]) + rng.uniform(-3, 3, 3) * np.array([1, 1, 0])
+             + np.array([0, 0, rng.uniform(-3, 3)]))
+        a = rng.uniform(0.2, 1.0)
+        sets.append(BeamSet.from_triangles(rng.normal(size=(1, 8)), [(0, 1.0, 0)],
+                                           [(np.sin(a), 0.0, -np.cos(a))], [1.0], v[None]))
+    return concat(sets)
+
+
+def test_fp64_clamped_edge_slopes(orc):
+    """Edges parallel to the aperture frame's axes (axis_edge_triangles; the records carry the
+    clamped slopes): the plain FP64 kernel within 1e-9 per bin of the oracle."""
+    th, ph = _coarse()
+    bs = axis_edge_triangles()
+    recs, _ = lib.prepare_beams_host(bs, LAM)
+    assert np.sum(recs[:, 46:49] == 1e6) >= bs.n and np.sum(recs[:, 49:52] == 1e6) >= bs.n
+    ref, _ = orc.diff_main(th, ph, LAM, bs, BeamSet.empty(), nthreads=4)
+    got, _ = lib.diff_main(th, ph, LAM, bs, BeamSet.empty(), precision="fp64")
+    err = ampl_rel_err(got, ref)
+    assert err.max() <= 1e-9, summary(err)
+
+
 def test_fp64_exact_forward_with_clamped_k(orc):
     """The exact forward bins of on_axis_triangles, where the reference's value is set by its
     own rounding (the exact form factor differs from it by ~1e-5 there): the plain FP64 kernel
